@@ -1,0 +1,260 @@
+"""Named, prebuilt kernels: the benchmark workloads and the chains the test-suite runs.
+
+``build_catalog()`` generates every entry and compiles each into ``lib/libtf_<entry>.so``;
+``load(name)`` rebuilds the entry's descriptions (kernel names are deterministic) and binds the
+launcher from the prebuilt object. A missing object is an error, never a fallback.
+"""
+import concurrent.futures
+import os
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional
+
+from . import workloads as W
+from .common.aux import generate_tmp_matrix, generate_tmp_tensor
+from .common.basic_types import Addressing, FloatingPointType
+from .common.context import Context
+from .common.matrix.boundingbox import BoundingBox
+from .common.matrix.spp import ListSPP
+from .common.matrix.tensor import SubTensor, Tensor
+from .generators.descriptions import GemmDescr, MultilinearDescr
+from .generators.generator import Generator
+from .runtime import build as rt_build
+from .runtime import loader as rt_loader
+
+F64 = FloatingPointType.DOUBLE
+F32 = FloatingPointType.FLOAT
+
+
+@dataclass
+class Entry:
+  name: str
+  descrs: List
+  fp_type: FloatingPointType
+  tensors: Dict[str, Tensor]
+  emitter: str = 'auto'
+  schedule: Optional[object] = None
+  meta: Dict = field(default_factory=dict)
+  generator: Optional[Generator] = None
+
+  def make_generator(self):
+    ctx = Context('gfx950', 'hip', self.fp_type)
+    self.generator = Generator(self.descrs, ctx, emitter=self.emitter, schedule=self.schedule)
+    return self.generator
+
+
+_REGISTRY: Dict[str, Callable[[], Entry]] = {}
+
+
+def register(name):
+  def deco(fn):
+    _REGISTRY[name] = fn
+    return fn
+  return deco
+
+
+def names():
+  return sorted(_REGISTRY)
+
+
+def get(name) -> Entry:
+  entry = _REGISTRY[name]()
+  entry.name = name
+  entry.make_generator()
+  return entry
+
+
+def so_path(name):
+  return os.path.join(rt_build.LIB_DIR, f'libtf_{name}.so')
+
+
+def load(name):
+  """``(entry, kernel)`` for a prebuilt entry."""
+  entry = get(name)
+  entry.generator.generate()
+  kernel = rt_loader.load_kernels(so_path(name), [entry.generator])[0]
+  return entry, kernel
+
+
+def _build_one(name):
+  entry = get(name)
+  entry.generator.generate()
+  return rt_build.compile_generators([entry.generator], f'tf_{name}')
+
+
+def build_catalog(selection=None, jobs=None):
+  rt_build.build_runtime()
+  todo = list(selection) if selection else names()
+  jobs = jobs or min(8, os.cpu_count() or 1)
+  with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
+    return list(pool.map(_build_one, todo))
+
+
+# --------------------------------------------------------------- workloads
+
+def _wl_entry(w, emitter, schedule=None):
+  return Entry(name='', descrs=w.descrs, fp_type=w.fp_type, tensors=w.tensors, emitter=emitter,
+               schedule=schedule, meta=w.meta)
+
+
+for _emitter in ('plain',):
+  for _fp, _fpn in ((F64, 'f64'), (F32, 'f32')):
+    register(f'gemm_{_fpn}_{_emitter}')(
+      lambda fp=_fp, e=_emitter: _wl_entry(W.gemm(fp), e))
+    register(f'ader_{_fpn}_{_emitter}')(
+      lambda fp=_fp, e=_emitter: _wl_entry(W.ader_volume(fp), e))
+    register(f'ader_dense_{_fpn}_{_emitter}')(
+      lambda fp=_fp, e=_emitter: _wl_entry(W.ader_volume(fp, sparse_k=False), e))
+
+
+# --------------------------------------------------------------- test chains
+# Shapes follow the reference's examples (example/*.py) and test suites (tests/gemms/testsuites).
+
+def _t(shape, addr, bbox=None, **kw):
+  if bbox is not None:
+    bbox = BoundingBox(*bbox)
+  return Tensor(shape, addr, bbox, **kw)
+
+
+def _sub(t, bbox=None):
+  return SubTensor(t, BoundingBox(*bbox) if bbox is not None else None)
+
+
+def _gemm_entry(trans_a, trans_b, alpha, beta, fp=F64, m=24, n=9, k=20, addr_a=Addressing.NONE,
+                addr_b=Addressing.STRIDED, addr_c=Addressing.STRIDED, emitter='plain'):
+  A = _t([k, m] if trans_a else [m, k], addr_a, alias='A')
+  B = _t([n, k] if trans_b else [k, n], addr_b, alias='B')
+  C = _t([m, n], addr_c, alias='C')
+  d = GemmDescr(trans_a, trans_b, _sub(A), _sub(B), _sub(C), alpha=alpha, beta=beta)
+  return Entry('', [d], fp, {'A': A, 'B': B, 'C': C}, emitter=emitter)
+
+
+for _ta in (False, True):
+  for _tb in (False, True):
+    register(f't_gemm_{"T" if _ta else "N"}{"T" if _tb else "N"}_plain')(
+      lambda ta=_ta, tb=_tb: _gemm_entry(ta, tb, 1.0, 1.0))
+register('t_gemm_alpha_beta_plain')(lambda: _gemm_entry(False, False, 1.5, 0.5))
+register('t_gemm_overwrite_f32_plain')(lambda: _gemm_entry(False, True, 1.0, 0.0, fp=F32))
+register('t_gemm_alpha_runtime_plain')(lambda: _gemm_entry(False, False, 'alpha', 1.0))
+register('t_gemm_ptr_plain')(lambda: _gemm_entry(False, False, 1.0, 1.0,
+                                                 addr_a=Addressing.PTR_BASED,
+                                                 addr_b=Addressing.PTR_BASED,
+                                                 addr_c=Addressing.PTR_BASED))
+register('t_gemm_strided_a_plain')(lambda: _gemm_entry(True, False, 1.0, 1.0,
+                                                       addr_a=Addressing.STRIDED, m=33, n=7, k=31))
+
+
+@register('t_bbox_plain')
+def _bbox_entry():
+  """Padded leading dims and sub-block windows (reference notrans-trans-tests.yaml:22-24)."""
+  A = _t([72, 40], Addressing.NONE, ([4, 2], [60, 30]), alias='A')
+  B = _t([33, 12], Addressing.STRIDED, ([2, 1], [30, 10]), alias='B')
+  C = _t([65, 11], Addressing.STRIDED, ([3, 0], [59, 9]), alias='C')
+  d = MultilinearDescr(_sub(C, ([4, 1], [60, 10])), [_sub(A, ([4, 2], [60, 30])),
+                                                     _sub(B, ([2, 1], [30, 10]))],
+                       [[0, -1], [-1, 1]], [[0, 1], [0, 1]], add=True)
+  return Entry('', [d], F64, {'A': A, 'B': B, 'C': C})
+
+
+@register('t_three_index_plain')
+def _three_index_entry():
+  """C_ijk = A_il · B_ljk (a 3-index contraction of the cost-model workload)."""
+  A = _t([12, 10], Addressing.NONE, alias='A')
+  B = _t([10, 6, 5], Addressing.STRIDED, alias='B')
+  C = _t([12, 6, 5], Addressing.STRIDED, alias='C')
+  d = MultilinearDescr(_sub(C), [_sub(A), _sub(B)], [[0, -1], [-1, 1, 2]],
+                       [[0, 1], [0, 1, 2]], add=False)
+  return Entry('', [d], F64, {'A': A, 'B': B, 'C': C})
+
+
+@register('t_sparse_dense_plain')
+def _sparse_dense_entry():
+  """Sparse × dense with a random 25% pattern (reference sparse_dense_gemms/test_loader.py:42)."""
+  import numpy as np
+  rng = np.random.default_rng(7)
+  mask = rng.random((40, 30)) < 0.25
+  A = _t([40, 30], Addressing.STRIDED, spp=ListSPP(W.csr_entries(mask), (40, 30)), alias='A')
+  B = _t([30, 9], Addressing.STRIDED, alias='B')
+  C = _t([40, 9], Addressing.STRIDED, alias='C')
+  d = MultilinearDescr(_sub(C), [_sub(A), _sub(B)], [[0, -1], [-1, 1]], [[0, 1], [0, 1]], add=True)
+  return Entry('', [d], F64, {'A': A, 'B': B, 'C': C}, meta={'mask': mask})
+
+
+@register('t_dense_sparse_plain')
+def _dense_sparse_entry():
+  """Dense × sparse with a random 15% pattern (reference dense_sparse_gemms/test_loader.py)."""
+  import numpy as np
+  rng = np.random.default_rng(8)
+  mask = rng.random((20, 16)) < 0.15
+  A = _t([36, 20], Addressing.STRIDED, alias='A')
+  B = _t([20, 16], Addressing.NONE, spp=ListSPP(W.csr_entries(mask), (20, 16)), alias='B')
+  C = _t([36, 16], Addressing.STRIDED, alias='C')
+  d = MultilinearDescr(_sub(C), [_sub(A), _sub(B)], [[0, -1], [-1, 1]], [[0, 1], [0, 1]])
+  return Entry('', [d], F64, {'A': A, 'B': B, 'C': C}, meta={'mask': mask})
+
+
+@register('t_four_matrices_plain')
+def _four_matrices_entry():
+  """Q += A x ((B x C) x D), all pointer-based (reference example/four_matrices.py:10-34)."""
+  q = _t([56, 56], Addressing.PTR_BASED, ([0, 0], [56, 9]), alias='Q')
+  a = _t([56, 56], Addressing.PTR_BASED, ([0, 0], [56, 20]), alias='A')
+  b = _t([56, 56], Addressing.PTR_BASED, ([0, 0], [20, 56]), alias='B')
+  c = _t([56, 9], Addressing.PTR_BASED, ([0, 0], [56, 9]), alias='C')
+  d = _t([9, 9], Addressing.PTR_BASED, ([0, 0], [9, 9]), alias='D')
+  mq, ma, mb, mc, md = (_sub(x, y) for x, y in ((q, ([0, 0], [56, 9])), (a, ([0, 0], [56, 20])),
+                                             (b, ([0, 0], [20, 56])), (c, ([0, 0], [56, 9])),
+                                             (d, ([0, 0], [9, 9]))))
+  tmp1 = generate_tmp_matrix(mb, mc)
+  tmp2 = generate_tmp_matrix(tmp1, md)
+  descrs = [GemmDescr(False, False, mb, mc, _sub(tmp1)),
+            GemmDescr(False, False, _sub(tmp1), md, _sub(tmp2)),
+            GemmDescr(False, False, ma, _sub(tmp2), mq, alpha=1.0, beta=1.0)]
+  return Entry('', descrs, F32, {'Q': q, 'A': a, 'B': b, 'C': c, 'D': d})
+
+
+@register('t_five_multiplies_plain')
+def _five_entry():
+  """Q = (((A x B) x (C x B))ᵀ-chain x D) (reference example/five_multiplies.py:10-39)."""
+  mq = _sub(_t([9, 9], Addressing.STRIDED, alias='Q'))
+  ma = _sub(_t([56, 56], Addressing.STRIDED, alias='A'))
+  mb = _sub(_t([56, 9], Addressing.STRIDED, alias='B'))
+  mc = _sub(_t([56, 56], Addressing.STRIDED, alias='C'))
+  md = _sub(_t([9, 9], Addressing.STRIDED, alias='D'))
+  tmp0 = generate_tmp_matrix(ma, mb)
+  tmp1 = generate_tmp_matrix(mc, mb)
+  tmp2 = generate_tmp_matrix(tmp0, tmp1, trans_a=True)
+  descrs = [GemmDescr(False, False, ma, mb, _sub(tmp0)),
+            GemmDescr(False, False, mc, mb, _sub(tmp1)),
+            GemmDescr(True, False, _sub(tmp0), _sub(tmp1), _sub(tmp2)),
+            GemmDescr(False, False, _sub(tmp2), md, mq, alpha=1.0, beta=0.0)]
+  return Entry('', descrs, F32, {'Q': mq.tensor, 'A': ma.tensor, 'B': mb.tensor, 'C': mc.tensor,
+                                 'D': md.tensor})
+
+
+@register('t_mat_vec_plain')
+def _mat_vec_entry():
+  """C += A x b (reference example/mat_vec.py:9-20)."""
+  c = _t([56, 1], Addressing.STRIDED, alias='C')
+  a = _t([56, 9], Addressing.STRIDED, alias='A')
+  b = _t([9, 1], Addressing.STRIDED, alias='B')
+  d = GemmDescr(False, False, _sub(a), _sub(b), _sub(c), alpha=1.0, beta=1.0)
+  return Entry('', [d], F32, {'A': a, 'B': b, 'C': c})
+
+
+@register('t_csa_plain')
+def _csa_entry():
+  """C = 13 · A x B (reference example/csa.py:29-46)."""
+  a = _t([9, 9], Addressing.STRIDED, alias='A')
+  b = _t([9, 9], Addressing.STRIDED, alias='B')
+  c = _t([9, 9], Addressing.STRIDED, alias='C')
+  d = GemmDescr(False, False, _sub(a), _sub(b), _sub(c), alpha=13.0, beta=0.0)
+  return Entry('', [d], F32, {'A': a, 'B': b, 'C': c})
+
+
+@register('t_lane_wide_plain')
+def _lane_wide_entry():
+  """Leading extents above one wavefront and small ones (order 8 and order 2 bases)."""
+  A = _t([120, 120], Addressing.NONE, alias='A')
+  B = _t([120, 9], Addressing.STRIDED, alias='B')
+  C = _t([120, 9], Addressing.STRIDED, alias='C')
+  d = MultilinearDescr(_sub(C), [_sub(A), _sub(B)], [[0, -1], [-1, 1]], [[0, 1], [0, 1]], add=True)
+  return Entry('', [d], F64, {'A': A, 'B': B, 'C': C})

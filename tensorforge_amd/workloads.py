@@ -1,0 +1,146 @@
+"""Description lists and seeded inputs of the benchmark workloads.
+
+* ``gemm``: ``C_e += A · B_e``, A b×b shared, B_e and C_e b×q per element (single batched GEMM).
+* ``ader_volume``: ``Q'_e = Σ_d K_d · Q_e · S_{e,d}``, d = 1..3: K_d b×b shared (sparse in CSR
+  value order, or dense), Q_e b×q, S_{e,d} a q×q per-element star matrix with a fixed 24-entry
+  pattern. This is the volume chain of SeisSol's ADER-DG step as written in the reference's
+  benchmark program ``benchmarks/fused/programs/s3time.cf:43``.
+
+Inputs come from ``numpy.random.default_rng(seed)`` (PCG64) in a fixed draw order, column-major
+per element, elements contiguous.
+"""
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from .common.aux import generate_tmp_tensor
+from .common.basic_types import Addressing, FloatingPointType
+from .common.matrix.spp import ListSPP
+from .common.matrix.tensor import SubTensor, Tensor
+from .generators.descriptions import MultilinearDescr
+
+# Elastic-wave star matrix pattern (quantities σxx σyy σzz σxy σyz σxz u v w = 0..8): the
+# stress rows couple to velocities, the velocity rows to stresses.
+STAR_PATTERN = ([(r, c) for r in range(3) for c in range(6, 9)] +
+                [(3, 6), (3, 7), (4, 7), (4, 8), (5, 6), (5, 8)] +
+                [(6, 0), (6, 3), (6, 5), (7, 3), (7, 1), (7, 4), (8, 5), (8, 4), (8, 2)])
+
+BASIS_SIZES = {1: 1, 2: 4, 3: 10, 4: 20, 5: 35, 6: 56, 7: 84, 8: 120}
+
+
+def star_entries():
+  """Star pattern entries in column-major order (the packed value order)."""
+  return sorted(STAR_PATTERN, key=lambda rc: (rc[1], rc[0]))
+
+
+@dataclass
+class Workload:
+  name: str
+  descrs: List
+  fp_type: FloatingPointType
+  tensors: Dict[str, Tensor]
+  # names of the launcher operands in launcher order are known after Generator naming
+  meta: Dict = field(default_factory=dict)
+
+
+def gemm(fp_type=FloatingPointType.DOUBLE, b=56, q=9) -> Workload:
+  A = Tensor([b, b], Addressing.NONE, alias='A')
+  B = Tensor([b, q], Addressing.STRIDED, alias='B')
+  C = Tensor([b, q], Addressing.STRIDED, alias='C')
+  d = MultilinearDescr(SubTensor(C), [SubTensor(A), SubTensor(B)], [[0, -1], [-1, 1]],
+                       [[0, 1], [0, 1]], add=True)
+  return Workload('gemm', [d], fp_type, {'A': A, 'B': B, 'C': C}, {'b': b, 'q': q})
+
+
+def k_patterns(b=56, density=0.4, seed=1, count=3):
+  rng = np.random.default_rng(seed)
+  masks = rng.random((count, b, b)) < density
+  return [m for m in masks]
+
+
+def csr_entries(mask):
+  """Nonzeros of ``mask`` in row-major (CSR) order."""
+  rows, cols = np.nonzero(mask)
+  return [(int(r), int(c)) for r, c in zip(rows, cols)]
+
+
+def ader_volume(fp_type=FloatingPointType.DOUBLE, b=56, q=9, sparse_k=True, density=0.4,
+                pattern_seed=1, star=True) -> Workload:
+  Q = Tensor([b, q], Addressing.STRIDED, alias='Q')
+  Qp = Tensor([b, q], Addressing.STRIDED, alias='Qp')
+  tensors = {'Q': Q, 'Qp': Qp}
+  masks = k_patterns(b, density, pattern_seed) if sparse_k else [None] * 3
+  descrs = []
+  for d in range(3):
+    if star:
+      S = Tensor([q, q], Addressing.STRIDED, spp=ListSPP(star_entries(), (q, q)), alias=f'S{d + 1}')
+    else:
+      S = Tensor([q, q], Addressing.STRIDED, alias=f'S{d + 1}')
+    if masks[d] is not None:
+      K = Tensor([b, b], Addressing.NONE, spp=ListSPP(csr_entries(masks[d]), (b, b)),
+                 alias=f'K{d + 1}')
+    else:
+      K = Tensor([b, b], Addressing.NONE, alias=f'K{d + 1}')
+    tensors[f'S{d + 1}'] = S
+    tensors[f'K{d + 1}'] = K
+    qs, ss, ks = SubTensor(Q), SubTensor(S), SubTensor(K)
+    T = generate_tmp_tensor([qs, ss], [[0, -1], [-1, 1]])
+    descrs.append(MultilinearDescr(SubTensor(T), [qs, ss], [[0, -1], [-1, 1]], [[0, 1], [0, 1]]))
+    descrs.append(MultilinearDescr(SubTensor(Qp), [ks, SubTensor(T)], [[0, -1], [-1, 1]],
+                                   [[0, 1], [0, 1]], add=d > 0))
+  return Workload('ader_volume', descrs, fp_type, tensors,
+                  {'b': b, 'q': q, 'sparse_k': sparse_k, 'masks': masks})
+
+
+# ---------------------------------------------------------------- inputs
+
+def tensor_storage_size(t: Tensor):
+  """Entries one element occupies in its array: the dense volume (also for packed sparse
+  tensors, whose values sit at the start of each element's slot)."""
+  return t.get_real_volume()
+
+
+def packed_count(t: Tensor):
+  return t.spp.count_nz() if t.is_sparse() else t.get_real_volume()
+
+
+def gemm_inputs(w: Workload, num_elements, seed=0, dtype=np.float64):
+  rng = np.random.default_rng(seed)
+  b, q = w.meta['b'], w.meta['q']
+  A = rng.uniform(-1, 1, size=b * b)
+  B = rng.uniform(-1, 1, size=num_elements * b * q)
+  C = rng.uniform(-1, 1, size=num_elements * b * q)
+  return {'A': A.astype(dtype), 'B': B.astype(dtype), 'C': C.astype(dtype)}
+
+
+def ader_inputs(w: Workload, num_elements, seed=2, dtype=np.float64, q_scale=None):
+  """Q ~ N(0,1) (or log-uniform magnitudes when ``q_scale=(lo_exp, hi_exp)``), star values
+  U[-2,2] packed at the start of each 81-entry slot, K values U[-1,1] packed in CSR order."""
+  rng = np.random.default_rng(seed)
+  b, q = w.meta['b'], w.meta['q']
+  out = {}
+  for d in range(3):
+    K = w.tensors[f'K{d + 1}']
+    out[f'K{d + 1}'] = rng.uniform(-1, 1, size=packed_count(K)).astype(dtype)
+  if q_scale is None:
+    out['Q'] = rng.standard_normal(num_elements * b * q).astype(dtype)
+  else:
+    lo, hi = q_scale
+    mag = 10.0 ** rng.uniform(lo, hi, size=num_elements * b * q)
+    sign = np.where(rng.random(num_elements * b * q) < 0.5, -1.0, 1.0)
+    out['Q'] = (mag * sign).astype(dtype)
+  for d in range(3):
+    S = w.tensors[f'S{d + 1}']
+    nnz = packed_count(S)
+    vals = np.zeros((num_elements, tensor_storage_size(S)), dtype=dtype)
+    vals[:, :nnz] = rng.uniform(-2, 2, size=(num_elements, nnz))
+    out[f'S{d + 1}'] = vals.reshape(-1)
+  out['Qp'] = np.zeros(num_elements * b * q, dtype=dtype)
+  return out
+
+
+def launcher_args_by_alias(generator):
+  """Launcher operands in order as aliases (``None`` for non-tensor arguments)."""
+  return [(name, next((t.alias for t in generator._global_tensors if t.name == name.split('_')[0]), None))
+          for _, name in generator.launcher_signature()]
