@@ -1,0 +1,277 @@
+"""Benchmark: the fused ADER-DG volume chain on one MI355X.
+
+Workload (the headline configuration): ``Q'_e = sum_d K_d Q_e S_{e,d}``, d = 1..3, order 6
+(56 basis functions x 9 quantities), 262,144 elements, FP64; K_d 56x56 sparse (~40% random
+pattern, CSR values), S_{e,d} 9x9 star matrices (24 nonzeros, packed). One step = one launch of
+the generated launcher over the whole batch, inputs resident in HBM.
+
+Prints one JSON line: elements/s of the tuned kernel, the plain (reference-mapping) kernel's
+elements/s on the same inputs, the HBM roofline of the tuned kernel (algorithmic bytes per launch
+over its average duration, from HIP events on the launch stream), the FP32 variant, and the CPU
+baseline (the OpenMP restatement of the reference's generated kernel, on a bounded sample, with
+the GPU result's error against it).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+  ap = argparse.ArgumentParser()
+  ap.add_argument('--gpus', type=int, default=1)
+  ap.add_argument('--steps', type=int, default=50)
+  ap.add_argument('--warmup', type=int, default=5)
+  ap.add_argument('--elements', type=int, default=262144)
+  ap.add_argument('--no-cpu', action='store_true')
+  ap.add_argument('--cpu-seconds', type=float, default=15.0)
+  ap.add_argument('--only', default='', help='comma list of legs (tuned,plain,f32,copy)')
+  return ap.parse_args()
+
+
+def make_inputs(entry, n, device, dtype, seed=2):
+  """Seeded device inputs in the launcher layout (values as in tensorforge_amd.workloads)."""
+  import torch
+  from tensorforge_amd import workloads as W
+  g = torch.Generator(device=device)
+  g.manual_seed(seed)
+  b, q = entry.meta['b'], entry.meta['q']
+  bufs = {}
+  for d in range(3):
+    K = entry.tensors[f'K{d + 1}']
+    nnz = W.packed_count(K)
+    bufs[f'K{d + 1}'] = (torch.rand(nnz, generator=g, device=device, dtype=torch.float64) * 2 - 1).to(dtype)
+  bufs['Q'] = torch.randn(n * b * q, generator=g, device=device, dtype=torch.float64).to(dtype)
+  nnz_s = W.packed_count(entry.tensors['S1'])
+  for d in range(3):
+    s = torch.zeros(n, q * q, device=device, dtype=dtype)
+    s[:, :nnz_s] = (torch.rand(n, nnz_s, generator=g, device=device, dtype=torch.float64) * 4 - 2).to(dtype)
+    bufs[f'S{d + 1}'] = s.reshape(-1)
+  bufs['Qp'] = torch.zeros(n * b * q, device=device, dtype=dtype)
+  return bufs
+
+
+def time_kernel(kernel, bufs, n, steps, warmup):
+  """Average launch duration (s) over ``steps`` launches, HIP events on the launch stream."""
+  import torch
+  from tensorforge_amd.device import launch_args
+  stream = torch.cuda.current_stream()
+  args = launch_args(kernel.generator, bufs, n, stream=stream)
+  for _ in range(warmup):
+    kernel(*args)
+  torch.cuda.synchronize()
+  start = torch.cuda.Event(enable_timing=True)
+  stop = torch.cuda.Event(enable_timing=True)
+  start.record(stream)
+  for _ in range(steps):
+    kernel(*args)
+  stop.record(stream)
+  torch.cuda.synchronize()
+  return start.elapsed_time(stop) / 1e3 / steps
+
+
+def copy_bandwidth(device, steps=20):
+  """Measured HBM roof: 2 * bytes / time of a 16-byte-vector copy of 2 GiB."""
+  import torch
+  from tensorforge_amd.runtime import runtime
+  rt = runtime()
+  n = (1 << 31) // 8
+  src = torch.empty(n, dtype=torch.float64, device=device).normal_()
+  dst = torch.empty_like(src)
+  stream = torch.cuda.current_stream()
+  for _ in range(3):
+    rt.tensorforge_copy(dst.data_ptr(), src.data_ptr(), n, stream.cuda_stream)
+  torch.cuda.synchronize()
+  start = torch.cuda.Event(enable_timing=True)
+  stop = torch.cuda.Event(enable_timing=True)
+  start.record(stream)
+  for _ in range(steps):
+    rt.tensorforge_copy(dst.data_ptr(), src.data_ptr(), n, stream.cuda_stream)
+  stop.record(stream)
+  torch.cuda.synchronize()
+  t = start.elapsed_time(stop) / 1e3 / steps
+  del src, dst
+  return 2 * n * 8 / t / 1e9
+
+
+def cpu_baseline(entry, bufs, gpu_out, budget_s):
+  """The OpenMP restatement of the reference's generated kernel (oracle/omp_chain.cpp, team
+  form and flat form) on the first elements of the same inputs, sized to ~budget_s of CPU work;
+  also the GPU result's max normwise error against it on that sample."""
+  import numpy as np
+  from tensorforge_amd import workloads as W
+  lib = ctypes.CDLL(os.path.join(ROOT, 'oracle', 'lib', 'libomp_chain.so'))
+  threads = lib.omp_max_threads()
+  b, q = entry.meta['b'], entry.meta['q']
+  host = {k: v.cpu().numpy().astype(np.float64) for k, v in bufs.items() if k != 'Qp'}
+  masks = entry.meta['masks']
+  rowptr, cols, vals = [], [], []
+  for d in range(3):
+    ents = W.csr_entries(masks[d])
+    rp = np.zeros(b + 1, dtype=np.int32)
+    for r, _ in ents:
+      rp[r + 1] += 1
+    rowptr.append(np.cumsum(rp).astype(np.int32))
+    cols.append(np.array([c for _, c in ents], dtype=np.int32))
+    vals.append(np.ascontiguousarray(host[f'K{d + 1}']))
+  star = W.star_entries()
+  srow = np.array([r for r, _ in star], dtype=np.int32)
+  scol = np.array([c for _, c in star], dtype=np.int32)
+  IP = ctypes.POINTER(ctypes.c_int)
+  DP = ctypes.POINTER(ctypes.c_double)
+  arr_i = (IP * 3)(*[x.ctypes.data_as(IP) for x in rowptr])
+  arr_c = (IP * 3)(*[x.ctypes.data_as(IP) for x in cols])
+  arr_v = (DP * 3)(*[x.ctypes.data_as(DP) for x in vals])
+  S = [np.ascontiguousarray(host[f'S{d + 1}']) for d in range(3)]
+  arr_s = (DP * 3)(*[x.ctypes.data_as(DP) for x in S])
+  Q = np.ascontiguousarray(host['Q'])
+
+  def run(form, n):
+    out = np.zeros(n * b * q)
+    t0 = time.perf_counter()
+    if form == 'team':
+      lib.omp_ader_team(arr_i, arr_c, arr_v, Q.ctypes.data_as(DP), arr_s, out.ctypes.data_as(DP),
+                        srow.ctypes.data_as(IP), scol.ctypes.data_as(IP), len(star), b, q,
+                        ctypes.c_long(n), 32, 8)
+    else:
+      lib.omp_ader_flat(arr_i, arr_c, arr_v, Q.ctypes.data_as(DP), arr_s, out.ctypes.data_as(DP),
+                        srow.ctypes.data_as(IP), scol.ctypes.data_as(IP), len(star), b, q,
+                        ctypes.c_long(n))
+    return time.perf_counter() - t0, out
+
+  result = {'cores': threads, 'kind': 'port'}
+  for form in ('team', 'flat'):
+    n = 256
+    t, _ = run(form, n)
+    while t < budget_s / 8 and n < 262144:
+      n = min(262144, n * 4)
+      t, _ = run(form, n)
+    n = int(min(262144, max(256, n * budget_s / 2 / max(t, 1e-6))) // 8 * 8)
+    t, out = run(form, n)
+    result[form] = {'value': n / t, 'elements': n, 'seconds': t}
+    if form == 'flat':
+      ref = out.reshape(n, b * q)
+      got = gpu_out[:n * b * q].reshape(n, b * q)
+      err = np.abs(got - ref).max(axis=1) / np.maximum(np.abs(ref).max(axis=1), 1e-300)
+      result['gpu_max_normwise_rel_err'] = float(err.max())
+  return result
+
+
+def main():
+  args = parse()
+  if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+    print('bench.py measures one GPU: this build has no multi-GPU path (elements are independent; '
+          'see DESIGN.md for how the batch would be split). Run with --gpus 1.', file=sys.stderr)
+    sys.exit(2)
+  import numpy as np
+  import torch
+  from tensorforge_amd import catalog
+  device = torch.device('cuda:0')
+  legs = set(args.only.split(',')) if args.only else {'tuned', 'plain', 'f32', 'copy'}
+  n = args.elements
+  out = {}
+
+  entry, tuned = catalog.load('ader_f64_tuned')
+  bufs = make_inputs(entry, n, device, torch.float64)
+  emitted = tuned.generator.get_emitted()
+  bytes_per_launch = emitted.bytes_per_element * n + emitted.shared_bytes
+  flops = tuned.generator.get_program().flops_per_element() * n
+  t_tuned = time_kernel(tuned, bufs, n, args.steps, args.warmup)
+  gpu_out = bufs['Qp'].cpu().numpy()
+  value = n / t_tuned
+  achieved = bytes_per_launch / t_tuned / 1e9
+
+  if 'plain' in legs:
+    _, plain = catalog.load('ader_f64_plain')
+    bufs_plain = dict(bufs)
+    bufs_plain['Qp'] = torch.zeros_like(bufs['Qp'])
+    t_plain = time_kernel(plain, bufs_plain, n, max(3, args.steps // 5), 2)
+    plain_out = bufs_plain['Qp'].cpu().numpy().reshape(n, -1)
+    got = gpu_out.reshape(n, -1)
+    agree = float((np.abs(got - plain_out).max(axis=1) /
+                   np.maximum(np.abs(plain_out).max(axis=1), 1e-300)).max())
+    out['plain'] = {'value': n / t_plain, 'ms_per_step': t_plain * 1e3,
+                    'tuned_over_plain': t_plain / t_tuned,
+                    'max_normwise_rel_diff_tuned_vs_plain': agree}
+    del bufs_plain
+    _, plain_dense = catalog.load('ader_dense_f64_plain')
+    bufs_pd = dict(bufs)
+    # the dense-K plain kernel reads K column-major: same values, densified
+    from tensorforge_amd import workloads as W
+    for d in range(3):
+      dense = torch.zeros(56 * 56, dtype=torch.float64, device=device)
+      ents = W.csr_entries(entry.meta['masks'][d])
+      idx = torch.tensor([r + 56 * c for r, c in ents], device=device)
+      dense[idx] = bufs[f'K{d + 1}']
+      bufs_pd[f'K{d + 1}'] = dense
+    bufs_pd['Qp'] = torch.zeros_like(bufs['Qp'])
+    t_pd = time_kernel(plain_dense, bufs_pd, n, max(3, args.steps // 5), 2)
+    out['plain_dense_k'] = {'value': n / t_pd, 'ms_per_step': t_pd * 1e3,
+                            'tuned_over_plain_dense_k': t_pd / t_tuned}
+    del bufs_pd
+
+  if 'f32' in legs:
+    e32, k32 = catalog.load('ader_f32_tuned')
+    b32 = make_inputs(e32, n, device, torch.float32)
+    t32 = time_kernel(k32, b32, n, args.steps, args.warmup)
+    bpe32 = k32.generator.get_emitted().bytes_per_element
+    o32 = b32['Qp'].double().cpu().numpy().reshape(n, -1)
+    ref = gpu_out.reshape(n, -1)
+    err32 = float((np.abs(o32 - ref).max(axis=1) / np.maximum(np.abs(ref).max(axis=1), 1e-300)).max())
+    out['f32'] = {'value': n / t32, 'ms_per_step': t32 * 1e3,
+                  'hbm_GBps': (bpe32 * n + k32.generator.get_emitted().shared_bytes) / t32 / 1e9,
+                  'max_normwise_rel_err_vs_f64_kernel': err32}
+    del b32
+
+  measured_bw = copy_bandwidth(device) if 'copy' in legs else None
+
+  cpu = None
+  if not args.no_cpu:
+    cpu = cpu_baseline(entry, bufs, gpu_out, args.cpu_seconds)
+
+  line = {
+    'metric': 'elements/s + % HBM roofline, fused ADER chain, tuned vs plain at stated rel err',
+    'value': value,
+    'unit': 'elements/s',
+    'n_gpus': 1,
+    'steps': args.steps,
+    'warmup': args.warmup,
+    'ms_per_step': t_tuned * 1e3,
+    'higher_is_better': True,
+    'scaling': 'weak',
+    'vs_baseline': None,
+    'dtype': 'f64',
+    'data': 'synthetic (seeded: Q ~ N(0,1), S star U[-2,2], K CSR U[-1,1])',
+    'config': {'workload': 'fused ADER-DG volume chain Q\'=sum_d K_d Q S_d, order 6 '
+                           '(56x9), 262144 elements, K_d 56x56 ~40% sparse CSR, S_d 9x9 star '
+                           '(24 nnz), FP64', 'elements': n, 'emitter': 'tuned',
+               'schedule': emitted.schedule},
+    'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBPS, 'unit': 'GB/s',
+                 'frac': achieved / HBM_PEAK_GBPS, 'traffic': None,
+                 'bytes_per_element': emitted.bytes_per_element,
+                 'measured_copy_GBps': measured_bw,
+                 'frac_of_measured_copy': (achieved / measured_bw) if measured_bw else None,
+                 'fp64_tflops': flops / t_tuned / 1e12,
+                 'fp64_frac': flops / t_tuned / 1e12 / FP64_PEAK_TFLOPS},
+  }
+  line.update(out)
+  if cpu is not None:
+    line['cpu_baseline'] = {
+      'value': cpu['team']['value'], 'unit': 'elements/s', 'cores': cpu['cores'], 'kind': 'port',
+      'sample': (f"first {cpu['team']['elements']} elements of the same inputs, OpenMP team form "
+                 f"(reference omptarget structure: 32 lanes x 8 elements per team)"),
+      'flat_form': cpu['flat'],
+      'gpu_max_normwise_rel_err_vs_cpu': cpu.get('gpu_max_normwise_rel_err')}
+  print(json.dumps(line))
+
+
+if __name__ == '__main__':
+  main()

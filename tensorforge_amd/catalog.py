@@ -96,7 +96,7 @@ def _wl_entry(w, emitter, schedule=None):
                schedule=schedule, meta=w.meta)
 
 
-for _emitter in ('plain',):
+for _emitter in ('plain', 'tuned'):
   for _fp, _fpn in ((F64, 'f64'), (F32, 'f32')):
     register(f'gemm_{_fpn}_{_emitter}')(
       lambda fp=_fp, e=_emitter: _wl_entry(W.gemm(fp), e))
@@ -141,6 +141,18 @@ register('t_gemm_ptr_plain')(lambda: _gemm_entry(False, False, 1.0, 1.0,
                                                  addr_c=Addressing.PTR_BASED))
 register('t_gemm_strided_a_plain')(lambda: _gemm_entry(True, False, 1.0, 1.0,
                                                        addr_a=Addressing.STRIDED, m=33, n=7, k=31))
+
+
+def _auto_variants():
+  """Every test chain again with the default emitter choice (tuned where it applies)."""
+  for name in [n for n in list(_REGISTRY) if n.startswith('t_') and n.endswith('_plain')]:
+    base = _REGISTRY[name]
+
+    def make(base=base):
+      entry = base()
+      entry.emitter = 'auto'
+      return entry
+    _REGISTRY[name[:-len('_plain')] + '_auto'] = make
 
 
 @register('t_bbox_plain')
@@ -258,3 +270,6 @@ def _lane_wide_entry():
   C = _t([120, 9], Addressing.STRIDED, alias='C')
   d = MultilinearDescr(_sub(C), [_sub(A), _sub(B)], [[0, -1], [-1, 1]], [[0, 1], [0, 1]], add=True)
   return Entry('', [d], F64, {'A': A, 'B': B, 'C': C})
+
+
+_auto_variants()

@@ -26,7 +26,7 @@ def make_inputs(entry, num_elements, seed=0, offsets=None, garbage_unused=True):
   offsets = dict(offsets or {})
   host = {}
   for t in gen._global_tensors:
-    key = t.alias
+    key = key_of(t)
     vol = t.get_real_volume()
     if t.addressing == Addressing.SCALAR:
       if t.data is None:
@@ -48,18 +48,23 @@ def oracle_outputs(entry, host, num_elements, offsets=None, flags=None):
   gen = entry.generator
   arrays = {}
   for t in gen._global_tensors:
-    if t.alias in host:
-      v = host[t.alias]
+    if key_of(t) in host:
+      v = host[key_of(t)]
       arrays[t] = [x.astype(np.float64).copy() for x in v] if isinstance(v, list) else \
         (np.asarray(v, dtype=np.float64).copy() if not isinstance(v, float) else v)
-  offs = {t: (offsets or {}).get(t.alias, 0) for t in gen._global_tensors}
+  offs = {t: (offsets or {}).get(key_of(t), 0) for t in gen._global_tensors}
   inp = OracleInputs(arrays, num_elements, flags=flags, offsets=offs)
   out = oracle_run(entry.descrs, inp)
-  return {t.alias: out[t] for t in gen._global_tensors if t in out}
+  return {key_of(t): out[t] for t in gen._global_tensors if t in out}
+
+
+def key_of(t):
+  """Buffer key of a tensor: its alias, else its generated name."""
+  return t.alias if t.alias is not None else t.name
 
 
 def outputs_of(entry):
-  return [t.alias for t in entry.generator._global_tensors
+  return [key_of(t) for t in entry.generator._global_tensors
           if t.direction in (DataFlowDirection.SINK, DataFlowDirection.SOURCESINK)]
 
 

@@ -47,7 +47,7 @@ def test_thread_policy_values():
   # lanes per element and elements per block of the reference heuristic
   for b, lanes in ((56, 32), (84, 32), (4, 4), (10, 16), (20, 32), (35, 32)):
     w = W.gemm(b=b)
-    g = Generator(w.descrs, ctx())
+    g = Generator(w.descrs, ctx(), emitter='plain')
     g.generate()
     assert g.get_emitted().block[0] == lanes, b
   w = W.gemm()

@@ -9,7 +9,7 @@ import pytest
 from tensorforge_amd import catalog
 from tensorforge_amd.common.basic_types import Addressing
 
-from harness import TOL, gpu_run, make_inputs, normwise_error, oracle_outputs, outputs_of
+from harness import TOL, gpu_run, key_of, make_inputs, normwise_error, oracle_outputs, outputs_of
 
 pytestmark = pytest.mark.gpu
 
@@ -24,17 +24,19 @@ def _check(entry, kernel, device, num_elements, seed=0, offsets=None, flags=None
   got = gpu_run(kernel, entry, host, num_elements, device, offsets=offsets, flags=flags)
   tol = TOL[entry.fp_type]
   for t in entry.generator._global_tensors:
-    if t.alias not in outputs_of(entry) or t.alias not in got:
+    key = key_of(t)
+    if key not in outputs_of(entry) or key not in got:
       continue
-    off = (offsets or {}).get(t.alias, 0)
-    if isinstance(ref[t.alias], list):
-      err = normwise_error(got[t.alias], [r[off:] for r in ref[t.alias]], t.get_real_volume())
+    off = (offsets or {}).get(key, 0)
+    if isinstance(ref[key], list):
+      err = normwise_error([g[off:] for g in got[key]], [r[off:] for r in ref[key]],
+                           t.get_real_volume())
     else:
-      err = normwise_error(got[t.alias], ref[t.alias], t.get_real_volume(), off)
+      err = normwise_error(got[key], ref[key], t.get_real_volume(), off)
       # entries outside the written elements are untouched
-      np.testing.assert_array_equal(got[t.alias][:off], host[t.alias][:off])
-    print(f'{entry.name} {t.alias} N={num_elements} normwise rel err {err:.3e} (tol {tol})')
-    assert err <= tol, f'{entry.name}: {t.alias} normwise error {err:.3e} > {tol}'
+      np.testing.assert_array_equal(got[key][:off], host[key][:off])
+    print(f'{entry.name} {key} N={num_elements} normwise rel err {err:.3e} (tol {tol})')
+    assert err <= tol, f'{entry.name}: {key} normwise error {err:.3e} > {tol}'
 
 
 @pytest.mark.parametrize('name', PLAIN_TESTS)
