@@ -65,6 +65,9 @@ class Schedule:
   fuse_temporaries: bool = True
   # minimum waves per SIMD requested from the register allocator (launch bounds hint)
   min_waves_per_simd: int = 1
+  # workgroup barrier before each uniform-left product: keeps the wavefronts of a workgroup in
+  # the same phase so they share the scalar cache's copy of the uniform operand
+  phase_sync: bool = False
 
   def key(self):
     return tuple(sorted(asdict(self).items()))
@@ -275,8 +278,9 @@ class _Emitter:
       w(f'extern __shared__ __align__(16) {real} lds_all[];')
       w(f'{real}* __restrict__ lds = lds_all + {self.elem_stride * self.G} * (threadIdx.x / {WAVE});')
     w(f'const int lane = threadIdx.x % {WAVE};')
-    w(f'const int slot = lane / {self.q};')
-    w(f'const int col = lane - slot * {self.q};')
+    # lanes past the last element slot work on a copy of the last slot's data (results unused)
+    w(f'const int slot = lane / {self.q} < {self.G} ? lane / {self.q} : {self.G - 1};')
+    w(f'const int col = lane / {self.q} < {self.G} ? lane - slot * {self.q} : 0;')
     w(f'const size_t e0 = ((size_t)blockIdx.x * {waves} + threadIdx.x / {WAVE}) * {self.G};')
     w('const size_t batchId = e0 + slot;')
     w(f'const bool active = (lane < {self.G * self.q}) && (batchId < numElements) && '
@@ -574,6 +578,8 @@ class _Emitter:
       return
     if kind in ('lds', 'lds_tmp'):
       self.stage_left(p, shape)
+    if self.sched.phase_sync and kind in ('uniform', 'data'):
+      w('__syncthreads();')
     scale = self._scalar_expr(p)
     acc, state, rows = self.dest_acc(p, pi)
     zero = _lit(self.fp, 0)
