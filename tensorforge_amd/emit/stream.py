@@ -1,0 +1,540 @@
+"""Streaming schedule of the tuned emitter: persistent wavefronts, shared operands in LDS,
+DPP-broadcast FMAs.
+
+Applies to chains of the form (SeisSol's volume and local-flux chains)::
+
+    D_e = [beta D_e]  +  sum_t  U_t · R_t,e            (one destination, one column per lane)
+    R_t,e = L_e · W_t,e     (a temporary, formed on the fly)   or   R_t,e = B_t,e (an operand)
+
+with every ``U_t`` shared by all elements (``Addressing.NONE``, dense or sparse) and ``L_e`` one
+per-element operand shared by all temporaries.
+
+Mapping. A workgroup of ``W`` wavefronts stays resident and walks element groups (``64 // q``
+elements per wavefront). At start it copies the stored entries of every ``U_t`` into LDS in the
+order the FMAs consume them. Per element group, the contraction runs in blocks of ``k_block``
+rows: the block's rows of ``L_e`` are loaded one block ahead into registers and written to a
+two-slot LDS ring; each lane forms its temporary's rows from the ring (a short per-lane list of
+(column, weight) slots for a sparse ``W``), then every stored entry of ``U_t`` costs one
+``v_fmac_f64_dpp`` whose shared operand comes from a 16-entry LDS chunk broadcast across each row
+of 16 lanes (``row_newbcast``). No scalar loads and no lane branches are in the loop; the
+destination column stays in registers and is written once per element.
+"""
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+from ..common.basic_types import Addressing, FloatingPointType
+from . import ir
+from .launcher import Emitted, Writer, kernel_params
+from .traffic import element_traffic
+
+WAVE = 64
+CHUNK = 16
+
+
+@dataclass
+class Term:
+  product: ir.Product
+  U: ir.Factor
+  u_dims: tuple                 # storage dims (row, k) of U
+  kind: str                     # 'fused' or 'global'
+  producer: Optional[ir.Product] = None
+  W: Optional[ir.Factor] = None     # weights of a fused temporary (right operand of producer)
+  w_dims: tuple = ()
+  B: Optional[ir.Factor] = None     # per-element right operand of a 'global' term
+  b_dims: tuple = ()
+  entries: list = field(default_factory=list)   # (row rel, k, position in U)
+
+
+@dataclass
+class Plan:
+  dest: ir.GlobalArg
+  dest_product: ir.Product
+  rows: tuple
+  cols: tuple
+  q: int
+  klo: int
+  khi: int
+  init: str                      # 'zero' or 'load'
+  beta: float
+  terms: List[Term]
+  L: Optional[ir.Factor] = None  # streamed per-element operand of fused terms
+  l_cols: tuple = ()             # column window of L
+
+
+def _classify(p):
+  from .tuned import _classify as c
+  return c(p)
+
+
+def plan(program: ir.ChainProgram):
+  """Recognise the streaming form; returns (Plan, None) or (None, reason)."""
+  prods = program.products
+  consumers = [p for p in prods if p.dest_src == ir.SRC_GLOBAL]
+  producers = [p for p in prods if p.dest_src == ir.SRC_TMP]
+  if not consumers:
+    return None, 'no global destination'
+  dest = consumers[0].dest_arg
+  rows, cols = tuple(consumers[0].n_ranges[0]), tuple(consumers[0].n_ranges[1])
+  q = cols[1] - cols[0]
+  if not 1 <= q <= WAVE // 2:
+    return None, f'{q} columns'
+  if rows[1] - rows[0] > 96:
+    return None, 'more than 96 rows'
+  by_tmp = {}
+  for p in producers:
+    if p.add or p.scalars:
+      return None, 'accumulating or scaled temporary'
+    sh, why = _classify(p)
+    if sh is None:
+      return None, why
+    if sh.left.src != ir.SRC_GLOBAL or not sh.left.arg.per_element or sh.left.is_sparse():
+      return None, 'temporary whose left operand is not a dense per-element operand'
+    if sh.left_dims != (0, 1):
+      return None, 'transposed left operand of a temporary'
+    if sh.right.src != ir.SRC_GLOBAL or not sh.right.arg.per_element:
+      return None, 'temporary whose right operand is not per element'
+    by_tmp[id(p.dest)] = (p, sh)
+  terms = []
+  L = None
+  l_window = None
+  klo = khi = None
+  init, beta = 'zero', 0.0
+  for n, p in enumerate(consumers):
+    if p.dest_arg is not dest or tuple(p.n_ranges[0]) != rows or tuple(p.n_ranges[1]) != cols:
+      return None, 'consumers write different destinations or windows'
+    if p.scalars:
+      return None, 'scaled product'
+    if n == 0:
+      if p.add:
+        init, beta = 'load', p.beta
+    elif not p.add or p.beta != 1.0:
+      return None, 'a later product overwrites the destination'
+    sh, why = _classify(p)
+    if sh is None:
+      return None, why
+    U = sh.left
+    if U.src != ir.SRC_GLOBAL or U.arg.addressing != Addressing.NONE:
+      return None, 'left operand is not shared by all elements'
+    if tuple(p.k_ranges[0]) != (klo, khi) and klo is not None:
+      return None, 'products with different contraction ranges'
+    klo, khi = p.k_ranges[0]
+    R = sh.right
+    if R.src == ir.SRC_TMP:
+      if id(R.tensor) not in by_tmp or sh.right_dims != (0, 1):
+        return None, 'temporary not formed by a streamable product'
+      prod, psh = by_tmp.pop(id(R.tensor))
+      if tuple(prod.n_ranges[0]) != (0, khi) or klo != 0:
+        return None, 'temporary rows do not match the contraction'
+      if prod.n_ranges[1][0] != 0 or prod.n_ranges[1][1] < q:
+        return None, 'temporary columns do not match'
+      if L is None:
+        L = psh.left
+        l_window = tuple(prod.k_ranges[0])
+      elif psh.left.arg is not L.arg or tuple(prod.k_ranges[0]) != l_window:
+        return None, 'temporaries stream different operands'
+      terms.append(Term(p, U, sh.left_dims, 'fused', producer=prod, W=psh.right,
+                        w_dims=psh.right_dims))
+    elif R.src == ir.SRC_GLOBAL and R.arg.per_element and not R.is_sparse():
+      terms.append(Term(p, U, sh.left_dims, 'global', B=R, b_dims=sh.right_dims))
+    else:
+      return None, 'right operand kind'
+  if by_tmp:
+    return None, 'temporary used other than by a single streamed product'
+  # every product must be a consumer or its producer
+  return Plan(dest=dest, dest_product=consumers[0], rows=rows, cols=cols, q=q, klo=klo, khi=khi,
+              init=init, beta=beta, terms=terms, L=L, l_cols=l_window or ()), None
+
+
+class _Stream:
+  def __init__(self, program, context, base_name, schedule, pl: Plan):
+    self.prog = program
+    self.ctx = context
+    self.fp = program.fp_type
+    self.real = FloatingPointType.as_str(self.fp)
+    self.sched = schedule
+    self.pl = pl
+    self.kname = f'kernel_{base_name}'
+    self.w = Writer()
+    self.q = pl.q
+    self.G = WAVE // pl.q
+    self.W = max(1, schedule.waves_per_block)
+    self.KB = max(2, schedule.k_block)
+    self.n = 0
+
+  def fresh(self, p):
+    self.n += 1
+    return f'{p}{self.n}'
+
+  def lit(self, v):
+    return self.fp.literal(v)
+
+  # ---------------------------------------------------------------- layout
+  def order_entries(self):
+    """Traversal order of every stored U entry: block, term, then k, row (columns of a block are
+    visited one after the other so consecutive FMAs update different rows)."""
+    pl = self.pl
+    lo0 = pl.rows[0]
+    seq = []
+    for t_idx, t in enumerate(pl.terms):
+      U = t.U
+      nd, kd = t.u_dims
+      if U.is_sparse():
+        items = U.packed.items()
+      else:
+        items = [(idx, U.offset(idx)) for idx in
+                 ir.iterate_box([(U.origin[d], U.origin[d] + U.tensor.shape[d]) for d in range(2)])]
+      for idx, pos in items:
+        i, k = idx[nd], idx[kd]
+        if pl.rows[0] <= i < pl.rows[1] and pl.klo <= k < pl.khi:
+          t.entries.append((i - lo0, k, pos))
+    blocks = list(range(pl.klo, pl.khi, self.KB))
+    for b0 in blocks:
+      # within a block: contraction index, then term, then row, so the right operand's row k
+      # of every term is consumed right after it is formed
+      es = [(e[1], t_idx, e[0], e) for t_idx, t in enumerate(pl.terms) for e in t.entries
+            if b0 <= e[1] < b0 + self.KB]
+      for _, t_idx, _, e in sorted(es, key=lambda x: x[:3]):
+        seq.append((b0, t_idx, e))
+    self.blocks = blocks
+    self.seq = seq
+    self.nu = len(seq)
+    self.nu_pad = (self.nu + CHUNK - 1) // CHUNK * CHUNK
+
+  # ---------------------------------------------------------------- emission
+  def emit(self) -> Emitted:
+    pl = self.pl
+    self.order_entries()
+    w = self.w
+    real = self.real
+    W, G, q, KB = self.W, self.G, self.q, self.KB
+    threads = WAVE * W
+    # ring of the streamed operand: per wavefront 2 slots x pairs x (KB + 1) rows
+    if pl.L is not None:
+      self.l_q = pl.l_cols[1] - pl.l_cols[0]
+      self.pairs = G * self.l_q
+      self.ps = KB + 1
+      self.slot_words = self.pairs * self.ps
+      ring = 2 * self.slot_words
+    else:
+      ring = 0
+    self.ring_words = ring
+    lds_words = self.nu_pad + W * ring
+    lds_bytes = lds_words * self.fp.size()
+    # tables: position of every traversal entry in its U array, term in the top bits
+    tab = ', '.join(f'{(t_idx << 20) | e[2]}u' for (_, t_idx, e) in self.seq)
+    w(f'static __device__ const unsigned {self.kname}_utab[{max(1, self.nu)}] = {{{tab or "0u"}}};')
+    params = ', '.join(kernel_params(self.prog, self.ctx))
+    mw = max(1, self.sched.stream_min_waves)
+    w(f'extern "C" __global__ void __launch_bounds__({threads}, {mw}) {self.kname}({params})')
+    w('{')
+    w._depth += 1
+    w(f'// streaming schedule: {W} wavefront(s) per workgroup, {G} elements x {q} columns per '
+      f'wavefront, contraction blocks of {KB}, {self.nu} shared entries in LDS')
+    w(f'extern __shared__ __align__(16) {real} lds[];')
+    w(f'{real}* __restrict__ ubuf = lds;')
+    w(f'const int wave = threadIdx.x / {WAVE};')
+    w(f'const int lane = threadIdx.x % {WAVE};')
+    if ring:
+      w(f'{real}* __restrict__ ring = lds + {self.nu_pad} + wave * {ring};')
+    w(f'const int slot = lane / {q} < {G} ? lane / {q} : {G - 1};')
+    w(f'const int col = lane / {q} < {G} ? lane - slot * {q} : 0;')
+    # stage the shared entries
+    with w.block(f'for (int e = threadIdx.x; e < {self.nu}; e += {threads})'):
+      w(f'const unsigned tv = {self.kname}_utab[e];')
+      w('const unsigned t = tv >> 20;')
+      w('const unsigned pos = tv & 0xfffffu;')
+      expr = f'({real}){pl.terms[-1].U.arg.name}[pos]'
+      for t_idx in range(len(pl.terms) - 2, -1, -1):
+        expr = f'(t == {t_idx} ? ({real}){pl.terms[t_idx].U.arg.name}[pos] : {expr})'
+      w(f'ubuf[e] = {expr};')
+    w('__syncthreads();')
+    self.per_lane_setup()
+    w(f'const size_t ngroups = (numElements + {G - 1}) / {G};')
+    w(f'const size_t stride = (size_t)gridDim.x * {W};')
+    w('const size_t rounds = (ngroups + stride - 1) / stride;')
+    with w.block('for (size_t r = 0; r < rounds; ++r)'):
+      w(f'const size_t e0 = ((r * gridDim.x + blockIdx.x) * {W} + wave) * {G};')
+      w('const size_t batchId = e0 + slot;')
+      w(f'const bool active = (lane < {G * q}) && (batchId < numElements) && '
+        '(flags == nullptr || flags[batchId] != 0u);')
+      w('const size_t eSafe = active ? batchId : 0;')
+      self.group_body()
+    w._depth -= 1
+    w('}')
+    traffic = element_traffic(self.prog)
+    sched = {'kind': 'stream', 'waves_per_block': W, 'k_block': KB, 'elements_per_wave': G,
+             'columns': q, 'shared_entries': self.nu, 'lds_bytes': lds_bytes,
+             'dpp': self.sched.dpp}
+    return Emitted(kernel_name=self.kname, kernel_src=w.src(), block=(threads, 1, 1),
+                   elements_per_block=G * W, lds_bytes=lds_bytes, emitter='tuned',
+                   schedule=sched, bytes_per_element=traffic.per_element,
+                   shared_bytes=traffic.shared,
+                   grid_override='tf_grid',
+                   extra_launch_setup=self.grid_setup(lds_bytes, threads))
+
+  def grid_setup(self, lds_bytes, threads):
+    G, W = self.G, self.W
+    return (f'static int tf_resident = 0;\n'
+            f'if (tf_resident == 0) {{\n'
+            f'  int dev = 0, cus = 0, per_cu = 0;\n'
+            f'  (void)hipGetDevice(&dev);\n'
+            f'  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);\n'
+            f'  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&{self.kname}), '
+            f'hipFuncAttributeMaxDynamicSharedMemorySize, {lds_bytes});\n'
+            f'  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, {self.kname}, {threads}, '
+            f'{lds_bytes});\n'
+            f'  tf_resident = (cus > 0 ? cus : 256) * (per_cu > 0 ? per_cu : 1);\n'
+            f'}}\n'
+            f'const size_t tf_groups = (numElements + {G * W - 1}) / {G * W};\n'
+            f'const unsigned tf_grid = static_cast<unsigned>(tf_groups < (size_t)tf_resident ? '
+            f'tf_groups : (size_t)tf_resident);')
+
+  def _select(self, values):
+    if all(v == values[0] for v in values):
+      return str(values[0])
+    expr = str(values[-1])
+    for c in range(len(values) - 2, -1, -1):
+      expr = f'(col == {c} ? {values[c]} : {expr})'
+    return expr
+
+  def per_lane_setup(self):
+    """Per-lane slot tables of the fused temporaries (columns of L and packed positions of W)."""
+    w = self.w
+    pl = self.pl
+    w('int lane15 = lane & 15;')
+    self.slots = {}
+    self.offsets = {}
+    for t_idx, t in enumerate(pl.terms):
+      if t.kind != 'fused':
+        continue
+      prod = t.producer
+      mlo, mhi = prod.k_ranges[0]
+      c_lo = prod.n_ranges[1][0]
+      rk, rn = t.w_dims
+      cols = []
+      for c in range(self.q):
+        ms = []
+        for m in range(mlo, mhi):
+          idx = [0, 0]
+          idx[rk], idx[rn] = m, c_lo + c
+          if t.W.is_sparse():
+            pos = t.W.packed.get(tuple(idx))
+          else:
+            pos = t.W.offset(idx)
+          if pos is not None:
+            ms.append((m, pos))
+        cols.append(ms)
+      nslots = max(len(c) for c in cols)
+      entries = []
+      for s in range(nslots):
+        ms = tuple(c[s][0] if s < len(c) else (c[0][0] if c else mlo) for c in cols)
+        ps = [c[s][1] if s < len(c) else -1 for c in cols]
+        v = self.fresh('sl')
+        w(f'const int {v}_p = {self._select(ps)};')
+        # ring offset of the slot's column within the lane's element: pair * (KB + 1); terms
+        # whose patterns pick the same columns share the offset, and so the LDS reads
+        if ms not in self.offsets:
+          o = self.fresh('off')
+          w(f'int {o} = (slot * {self.l_q} + ({self._select(list(ms))} - {self.pl.l_cols[0]})) * '
+            f'{self.ps};')
+          self.offsets[ms] = o
+        entries.append((v, self.offsets[ms]))
+      self.slots[t_idx] = entries
+
+  def group_body(self):
+    w = self.w
+    pl = self.pl
+    real = self.real
+    rows = pl.rows[1] - pl.rows[0]
+    # per-element pointers
+    for a in self.prog.args:
+      if a.kind != ir.SRC_GLOBAL or not a.per_element:
+        continue
+      ty = f'{"const " if a.const else ""}{a.dtype}'
+      if a.addressing == Addressing.STRIDED:
+        w(f'{ty}* __restrict__ g_{a.name} = {a.name} + eSafe * {a.element_stride}ull + '
+          f'{a.name}_extraOffset;')
+      else:
+        w(f'{ty}* __restrict__ g_{a.name} = {a.name}[eSafe] + {a.name}_extraOffset;')
+    # weights of the fused temporaries
+    for t_idx, slots in self.slots.items():
+      t = pl.terms[t_idx]
+      for v, _ in slots:
+        w(f'const {real} {v}_x = ({real})g_{t.W.arg.name}[{v}_p < 0 ? 0 : {v}_p];')
+        w(f'const {real} {v}_v = {v}_p >= 0 ? {v}_x : {self.lit(0)};')
+    # destination column
+    acc = 'acc'
+    w(f'{real} {acc}[{rows}];')
+    dp = pl.dest_product
+    o0, o1 = dp.dest_origin
+    s0, s1 = dp.dest_strides
+    for i in range(rows):
+      off = f'{(pl.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
+      if pl.init == 'load':
+        old = f'({real})g_{pl.dest.name}[{off}]'
+        w(f'{acc}[{i}] = {old if pl.beta == 1.0 else f"{self.lit(pl.beta)} * {old}"};')
+      else:
+        w(f'{acc}[{i}] = {self.lit(0)};')
+    nb = len(self.blocks)
+    if pl.L is not None:
+      self.load_block(0, stage=True)
+    self.global_rows = {0: self.load_global_rows(0)}
+    for bi, b0 in enumerate(self.blocks):
+      w(f'// block [{b0}, {min(b0 + self.KB, pl.khi)})')
+      if pl.L is not None and bi + 1 < nb:
+        self.load_block(bi + 1, stage=False)
+      if bi + 1 < nb:
+        self.global_rows[bi + 1] = self.load_global_rows(bi + 1)
+      self.compute_block(bi, b0, acc)
+      if pl.L is not None and bi + 1 < nb:
+        self.write_block(bi + 1)
+    with w.block('if (active)'):
+      for i in range(rows):
+        off = f'{(pl.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
+        w(f'g_{pl.dest.name}[{off}] = {acc}[{i}];')
+
+  # ---- ring of the streamed operand ----
+  def _block_rows(self, bi):
+    b0 = self.blocks[bi]
+    return b0, min(b0 + self.KB, self.pl.khi)
+
+  def load_block(self, bi, stage):
+    """Load the block's rows of L for every (element, column) pair of this wavefront into
+    registers (one value per lane per step); ``stage`` writes them to the ring right away."""
+    w = self.w
+    pl = self.pl
+    L = pl.L
+    b0, b1 = self._block_rows(bi)
+    nr = b1 - b0
+    total = self.pairs * nr
+    steps = (total + WAVE - 1) // WAVE
+    names = []
+    vol = L.arg.element_stride
+    for s in range(steps):
+      v = self.fresh('rg')
+      # flat index -> (pair, row): rows fastest so a lane's neighbours read neighbouring rows;
+      # lanes past the end repeat the last entry (same value to the same LDS word)
+      last = min(total - 1, s * WAVE + WAVE - 1)
+      if s * WAVE + WAVE - 1 < total:
+        w(f'const int {v}_i = lane + {s * WAVE};')
+      else:
+        w(f'const int {v}_i = lane + {s * WAVE} < {total} ? lane + {s * WAVE} : {total - 1};')
+      w(f'const int {v}_pr = {v}_i / {nr};')
+      w(f'const int {v}_k = {v}_i - {v}_pr * {nr};')
+      w(f'const int {v}_el = {v}_pr / {self.l_q};')
+      w(f'const int {v}_c = {v}_pr - {v}_el * {self.l_q};')
+      w(f'const size_t {v}_e = e0 + {v}_el < numElements ? e0 + {v}_el : 0;')
+      if L.arg.addressing == Addressing.STRIDED:
+        base = f'{L.arg.name} + {v}_e * {vol}ull + {L.arg.name}_extraOffset'
+      else:
+        base = f'{L.arg.name}[{v}_e] + {L.arg.name}_extraOffset'
+      goff = (f'({b0 - L.origin[0]} + {v}_k) * {L.strides[0]} + '
+              f'({pl.l_cols[0] - L.origin[1]} + {v}_c) * {L.strides[1]}')
+      w(f'const {self.real} {v} = ({self.real})({base})[{goff}];')
+      names.append(v)
+    self.pending = (bi, names)
+    if stage:
+      self.write_block(bi)
+
+  def write_block(self, bi):
+    w = self.w
+    pbi, names = self.pending
+    assert pbi == bi
+    b0, b1 = self._block_rows(bi)
+    nr = b1 - b0
+    ring_slot = bi % 2
+    for v in names:
+      w(f'ring[{ring_slot * self.slot_words} + {v}_pr * {self.ps} + {v}_k] = {v};')
+    # LDS accesses of one wavefront are performed in order; the fence keeps the compiler from
+    # moving the block's reads above these writes
+    w('__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");')
+    w('__builtin_amdgcn_wave_barrier();')
+    w('__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");')
+
+  def load_global_rows(self, bi):
+    """Rows of the block of every per-element right operand, loaded one block ahead."""
+    w = self.w
+    pl = self.pl
+    b0, b1 = self._block_rows(bi)
+    out = {}
+    for t_idx, t in enumerate(pl.terms):
+      if t.kind != 'global':
+        continue
+      B = t.B
+      kd, nd = t.b_dims
+      for k in range(b0, b1):
+        if not any(e[1] == k for e in t.entries):
+          continue
+        v = self.fresh('gr')
+        off = (f'({k - B.origin[kd]}) * {B.strides[kd]} + (col + {pl.cols[0] - B.origin[nd]}) '
+               f'* {B.strides[nd]}')
+        w(f'const {self.real} {v} = ({self.real})g_{B.arg.name}[{off}];')
+        out[(t_idx, k)] = v
+    return out
+
+  # ---- compute ----
+  def compute_block(self, bi, b0, acc):
+    w = self.w
+    pl = self.pl
+    real = self.real
+    b1 = min(b0 + self.KB, pl.khi)
+    ring_slot = bi % 2
+    block_seq = [(n, s) for n, s in enumerate(self.seq) if s[0] == b0]
+    fused_terms = [ti for ti, t in enumerate(pl.terms) if t.kind == 'fused']
+    offs = sorted(set(o for ti in fused_terms for _, o in self.slots[ti]))
+    # sequence point: the block's LDS reads use offsets rewritten here, so the compiler cannot
+    # hoist them above the previous block's FMAs (which would keep every operand live at once)
+    w(f'asm volatile("" : {", ".join(f"{chr(34)}+v{chr(34)}({o})" for o in ["lane15"] + offs)});')
+    chunks = {}
+    touched = []
+    for k in range(b0, b1):
+      col_entries = [(n, s) for n, s in block_seq if s[2][1] == k]
+      if not col_entries:
+        continue
+      rvals = {}
+      terms_here = sorted(set(s[1] for _, s in col_entries))
+      if any(pl.terms[ti].kind == 'fused' for ti in terms_here):
+        qv = {}
+        for o in offs:
+          v = self.fresh('q')
+          w(f'const {real} {v} = ring[{ring_slot * self.slot_words} + {o} + {k - b0}];')
+          qv[o] = v
+      for ti in terms_here:
+        t = pl.terms[ti]
+        if t.kind == 'fused':
+          slots = self.slots[ti]
+          v = self.fresh('r')
+          w(f'{real} {v} = {qv[slots[0][1]]} * {slots[0][0]}_v;')
+          for sv, o in slots[1:]:
+            w(f'{v} = __builtin_fma({qv[o]}, {sv}_v, {v});')
+          rvals[ti] = v
+        else:
+          rvals[ti] = self.global_rows[bi][(ti, k)]
+      for n, (_, ti, (i, kk, pos)) in col_entries:
+        c, j = divmod(n, CHUNK)
+        if c not in chunks:
+          cv = self.fresh('ch')
+          w(f'const {real} {cv} = ubuf[{c * CHUNK} + lane15];')
+          chunks[c] = cv
+        self.fma(f'{acc}[{i}]', chunks[c], j, rvals[ti])
+        if i not in touched:
+          touched.append(i)
+    for n0 in range(0, len(touched), 8):
+      ops = ', '.join(f'"+v"({acc}[{i}])' for i in touched[n0:n0 + 8])
+      w(f'asm volatile("" : {ops});')
+
+  def fma(self, acc, chunk, lane_in_row, r):
+    """acc += chunk[lane_in_row of the lane's row of 16] * r."""
+    w = self.w
+    ty = 'f64' if self.fp == FloatingPointType.DOUBLE else 'f32'
+    if self.sched.dpp == 'asm':
+      w(f'asm("v_fmac_{ty}_dpp %0, %1, %2 row_newbcast:{lane_in_row} row_mask:0xf bank_mask:0xf" '
+        f': "+v"({acc}) : "v"({chunk}), "v"({r}));')
+    else:
+      b = self.fresh('bc')
+      w(f'const {self.real} {b} = __builtin_amdgcn_mov_dpp({chunk}, {0x150 + lane_in_row}, 0xf, 0xf, true);')
+      w(f'{acc} = __builtin_fma({b}, {r}, {acc});')
+
+
+def emit(program, context, base_name, schedule, pl) -> Emitted:
+  return _Stream(program, context, base_name, schedule, pl).emit()

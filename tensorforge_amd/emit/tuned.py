@@ -68,6 +68,14 @@ class Schedule:
   # workgroup barrier before each uniform-left product: keeps the wavefronts of a workgroup in
   # the same phase so they share the scalar cache's copy of the uniform operand
   phase_sync: bool = False
+  # streaming schedule (persistent workgroups, shared operands in LDS, DPP-broadcast FMAs) when
+  # the chain has its form; its own workgroup size and contraction block
+  stream: bool = True
+  stream_waves: int = 4
+  stream_k_block: int = 8
+  stream_min_waves: int = 2
+  # how the DPP broadcast is issued: 'asm' (one v_fmac_*_dpp) or 'builtin' (v_mov_*_dpp + fma)
+  dpp: str = 'asm'
 
   def key(self):
     return tuple(sorted(asdict(self).items()))
@@ -807,4 +815,12 @@ class _Emitter:
 
 
 def emit(program, context, base_name, schedule=None) -> Emitted:
+  from . import stream
+  schedule = schedule or Schedule()
+  if schedule.stream:
+    pl, _ = stream.plan(program)
+    if pl is not None:
+      sched = Schedule(**{**asdict(schedule), 'waves_per_block': schedule.stream_waves,
+                          'k_block': schedule.stream_k_block})
+      return stream.emit(program, context, base_name, sched, pl)
   return _Emitter(program, context, base_name, schedule).emit()

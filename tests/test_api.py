@@ -167,3 +167,35 @@ def test_traffic_figures_of_workloads():
   g = Generator(W.ader_volume(FloatingPointType.FLOAT).descrs, ctx(FloatingPointType.FLOAT))
   g.generate()
   assert g.get_emitted().bytes_per_element == 4320
+
+
+def test_yateto_frontend_dictionary_gateway():
+  """add_tensor / add_operation / generate with yateto-like records (reference
+  frontend/yateto.py:214-253, 41-87, 151-170)."""
+  import tensorforge
+  fe_cls = tensorforge.get_routine_generator(None)
+
+  class Arch:
+    name, backend, typename = 'gfx950', 'hip', 'double'
+
+  fe = fe_cls(Arch())
+  for name, shape, addr, kind in (('A', [8, 8], '&', 'full'), ('B', [8, 3], 'n*N+o&', 'full'),
+                                  ('C', [8, 3], 'n&+o&', 'full')):
+    fe.add_tensor({'name': name, 'datatype': 'f64', 'addressing': addr,
+                   'storage': {'shape': shape, 'type': kind}, 'values': None,
+                   'flags': {'temporary': False, 'constant': False}})
+  ref = lambda n: {'name': n, 'spp': None}
+  fe.add_operation({'type': 'multilinear', 'result': ref('C'), 'args': [ref('A'), ref('B')],
+                    'condition': [], 'target': [[0, -1], [-1, 1]], 'permute': [[0, 1], [0, 1]],
+                    'linear': {'alpha': None, 'add': True}})
+  lines, routines = [], {}
+
+  class Cache:
+    def addRoutine(self, name, writer):
+      routines[name] = writer
+
+  fe.generate(lines.append, Cache())
+  assert len(routines) == 1
+  name = next(iter(routines))
+  assert lines[0] == (f'launcher_{name}(C, extraOffset_C, const_cast<const double*>(A), '
+                      'const_cast<const double*>(B), extraOffset_B, numElements, flags, streamPtr);')
