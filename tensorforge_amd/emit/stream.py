@@ -304,6 +304,7 @@ class _Stream:
     w('int lane15 = lane & 15;')
     self.slots = {}
     self.offsets = {}
+    self.positions = {}
     for t_idx, t in enumerate(pl.terms):
       if t.kind != 'fused':
         continue
@@ -330,7 +331,7 @@ class _Stream:
         ms = tuple(c[s][0] if s < len(c) else (c[0][0] if c else mlo) for c in cols)
         ps = [c[s][1] if s < len(c) else -1 for c in cols]
         v = self.fresh('sl')
-        w(f'const int {v}_p = {self._select(ps)};')
+        self.positions[v] = ps
         # ring offset of the slot's column within the lane's element: pair * (KB + 1); terms
         # whose patterns pick the same columns share the offset, and so the LDS reads
         if ms not in self.offsets:
@@ -356,10 +357,15 @@ class _Stream:
           f'{a.name}_extraOffset;')
       else:
         w(f'{ty}* __restrict__ g_{a.name} = {a.name}[eSafe] + {a.name}_extraOffset;')
-    # weights of the fused temporaries
+    # weights of the fused temporaries; their positions are recomputed from an opaque copy of
+    # the lane's column each group, so they do not occupy registers across the loop
+    if self.slots:
+      w('int colr = col;')
+      w('asm volatile("" : "+v"(colr));')
     for t_idx, slots in self.slots.items():
       t = pl.terms[t_idx]
       for v, _ in slots:
+        w(f'const int {v}_p = {self._select(self.positions[v]).replace("col ==", "colr ==")};')
         w(f'const {real} {v}_x = ({real})g_{t.W.arg.name}[{v}_p < 0 ? 0 : {v}_p];')
         w(f'const {real} {v}_v = {v}_p >= 0 ? {v}_x : {self.lit(0)};')
     # destination column
