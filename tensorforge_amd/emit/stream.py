@@ -200,10 +200,34 @@ class _Stream:
     self.nu = len(seq)
     self.nu_pad = (self.nu + CHUNK - 1) // CHUNK * CHUNK
 
+  def _loop_mode(self):
+    """Dense shared operands with many entries run the contraction blocks as a runtime loop
+    (the fully unrolled body would not fit the instruction cache)."""
+    pl = self.pl
+    if self.sched.stream_loop == 'never':
+      return False
+    if any(t.U.is_sparse() for t in pl.terms):
+      return False
+    if (pl.khi - pl.klo) % self.KB != 0:
+      return False
+    if self.sched.stream_loop == 'always':
+      return True
+    return self.nu > 4096
+
   # ---------------------------------------------------------------- emission
   def emit(self) -> Emitted:
     pl = self.pl
     self.order_entries()
+    self.loop = self._loop_mode()
+    self.rows_pad = (pl.rows[1] - pl.rows[0] + CHUNK - 1) // CHUNK * CHUNK
+    if self.loop:
+      # shared entries laid out column by column, each (column, term) padded to whole chunks
+      self.nu_pad = (pl.khi - pl.klo) * len(pl.terms) * self.rows_pad
+      self.u_source = self.sched.stream_u_source
+      if self.u_source == 'auto':
+        self.u_source = 'lds' if self.nu_pad * self.fp.size() <= 96 * 1024 else 'global'
+      if self.u_source == 'global':
+        self.nu_pad = 0
     w = self.w
     real = self.real
     W, G, q, KB = self.W, self.G, self.q, self.KB
@@ -221,8 +245,10 @@ class _Stream:
     lds_words = self.nu_pad + W * ring
     lds_bytes = lds_words * self.fp.size()
     # tables: position of every traversal entry in its U array, term in the top bits
-    tab = ', '.join(f'{(t_idx << 20) | e[2]}u' for (_, t_idx, e) in self.seq)
-    w(f'static __device__ const unsigned {self.kname}_utab[{max(1, self.nu)}] = {{{tab or "0u"}}};')
+    if not self.loop:
+      tab = ', '.join(f'{(t_idx << 20) | e[2]}u' for (_, t_idx, e) in self.seq)
+      w(f'static __device__ const unsigned {self.kname}_utab[{max(1, self.nu)}] = '
+        f'{{{tab or "0u"}}};')
     params = ', '.join(kernel_params(self.prog, self.ctx))
     mw = max(1, self.sched.stream_min_waves)
     w(f'extern "C" __global__ void __launch_bounds__({threads}, {mw}) {self.kname}({params})')
@@ -239,21 +265,43 @@ class _Stream:
     w(f'const int slot = lane / {q} < {G} ? lane / {q} : {G - 1};')
     w(f'const int col = lane / {q} < {G} ? lane - slot * {q} : 0;')
     # stage the shared entries
-    with w.block(f'for (int e = threadIdx.x; e < {self.nu}; e += {threads})'):
-      w(f'const unsigned tv = {self.kname}_utab[e];')
-      w('const unsigned t = tv >> 20;')
-      w('const unsigned pos = tv & 0xfffffu;')
-      expr = f'({real}){pl.terms[-1].U.arg.name}[pos]'
-      for t_idx in range(len(pl.terms) - 2, -1, -1):
-        expr = f'(t == {t_idx} ? ({real}){pl.terms[t_idx].U.arg.name}[pos] : {expr})'
-      w(f'ubuf[e] = {expr};')
+    if not self.loop:
+      with w.block(f'for (int e = threadIdx.x; e < {self.nu}; e += {threads})'):
+        w(f'const unsigned tv = {self.kname}_utab[e];')
+        w('const unsigned t = tv >> 20;')
+        w('const unsigned pos = tv & 0xfffffu;')
+        expr = f'({real}){pl.terms[-1].U.arg.name}[pos]'
+        for t_idx in range(len(pl.terms) - 2, -1, -1):
+          expr = f'(t == {t_idx} ? ({real}){pl.terms[t_idx].U.arg.name}[pos] : {expr})'
+        w(f'ubuf[e] = {expr};')
+    elif self.u_source == 'lds':
+      nt = len(pl.terms)
+      rp = self.rows_pad
+      nrows = pl.rows[1] - pl.rows[0]
+      with w.block(f'for (int e = threadIdx.x; e < {self.nu_pad}; e += {threads})'):
+        w(f'const int i = e % {rp};')
+        w(f'const int t = (e / {rp}) % {nt};')
+        w(f'const int k = e / {rp * nt} + {pl.klo};')
+        expr = self.lit(0)
+        for t_idx in range(nt - 1, -1, -1):
+          U = pl.terms[t_idx].U
+          nd, kd = pl.terms[t_idx].u_dims
+          pos = (f'(i + {pl.rows[0] - U.origin[nd]}) * {U.strides[nd]} + (k - {U.origin[kd]}) * '
+                 f'{U.strides[kd]}')
+          expr = f'(t == {t_idx} ? ({real}){U.arg.name}[{pos}] : {expr})'
+        w(f'ubuf[e] = i < {nrows} ? {expr} : {self.lit(0)};')
     w('__syncthreads();')
     self.per_lane_setup()
     w(f'const size_t ngroups = (numElements + {G - 1}) / {G};')
     w(f'const size_t stride = (size_t)gridDim.x * {W};')
     w('const size_t rounds = (ngroups + stride - 1) / stride;')
+    w(f'const size_t efirst = ((size_t)blockIdx.x * {W} + wave) * {G};')
+    w(f'const size_t estep = stride * {G};')
+    if not self.loop:
+      self.pipeline_prologue()
     with w.block('for (size_t r = 0; r < rounds; ++r)'):
-      w(f'const size_t e0 = ((r * gridDim.x + blockIdx.x) * {W} + wave) * {G};')
+      w('const size_t e0 = efirst + r * estep;')
+      w('const size_t e0n = e0 + estep;')
       w('const size_t batchId = e0 + slot;')
       w(f'const bool active = (lane < {G * q}) && (batchId < numElements) && '
         '(flags == nullptr || flags[batchId] != 0u);')
@@ -381,23 +429,316 @@ class _Stream:
         w(f'{acc}[{i}] = {old if pl.beta == 1.0 else f"{self.lit(pl.beta)} * {old}"};')
       else:
         w(f'{acc}[{i}] = {self.lit(0)};')
+    if self.loop:
+      self.loop_blocks(acc)
+      with w.block('if (active)'):
+        for i in range(rows):
+          off = f'{(pl.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
+          w(f'g_{pl.dest.name}[{off}] = {acc}[{i}];')
+      return
     nb = len(self.blocks)
-    if pl.L is not None:
-      self.load_block(0, stage=True)
-    self.global_rows = {0: self.load_global_rows(0)}
+    D = self.D
     for bi, b0 in enumerate(self.blocks):
       w(f'// block [{b0}, {min(b0 + self.KB, pl.khi)})')
-      if pl.L is not None and bi + 1 < nb:
-        self.load_block(bi + 1, stage=False)
-      if bi + 1 < nb:
-        self.global_rows[bi + 1] = self.load_global_rows(bi + 1)
+      # issue the loads of the block D ahead (this group's, or the next group's)
+      tb = bi + D
+      if tb < nb:
+        self.pipeline_load(tb, 'e0', D - 1)
+      else:
+        self.pipeline_load(tb - nb, 'e0n', D - 1)
+      self.global_rows = {bi: self.pipe_global[0]}
       self.compute_block(bi, b0, acc)
-      if pl.L is not None and bi + 1 < nb:
-        self.write_block(bi + 1)
+      # block bi + 1 (or the next group's first block) into the ring
+      nxt = bi + 1 if bi + 1 < nb else 0
+      self.pipeline_write(nxt)
+      self.pipeline_shift()
+    self.pipeline_realign()
     with w.block('if (active)'):
       for i in range(rows):
         off = f'{(pl.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
         w(f'g_{pl.dest.name}[{off}] = {acc}[{i}];')
+
+  # ---- cross-group prefetch pipeline (unrolled blocks) ----
+  def _pipe_steps(self, nr):
+    return (self.pairs * nr + WAVE - 1) // WAVE
+
+  def pipeline_prologue(self):
+    """Declare the pipeline registers and fill them with the first group's blocks 0..D-1;
+    block 0 goes to the ring."""
+    w = self.w
+    pl = self.pl
+    self.D = max(1, min(self.sched.stream_prefetch, len(self.blocks)))
+    D = self.D
+    nrmax = self.KB
+    self.pipe_names = []
+    for j in range(D):
+      if pl.L is not None:
+        w(f'{self.real} pf{j}[{self._pipe_steps(nrmax)}];')
+      self.pipe_names.append(f'pf{j}')
+    self.gterms = [ti for ti, t in enumerate(pl.terms) if t.kind == 'global']
+    for j in range(D):
+      for ti in self.gterms:
+        w(f'{self.real} pg{j}_{ti}[{nrmax}];')
+    # rotation state: python-side mapping of logical pipeline position -> register set name
+    self.pipe_rot = list(range(D))
+    self.pipe_block = [None] * D      # block index held by each logical position
+    w('{')
+    w._depth += 1
+    w('const size_t e0 = efirst;')
+    w('const size_t e0n = efirst + estep;')
+    for j in range(D):
+      if j < len(self.blocks):
+        self.pipeline_load(j, 'e0', j, declare=False)
+    w._depth -= 1
+    w('}')
+    self.pipeline_write(0)
+    self.pipeline_shift()
+    self.pipe_rot_entry = list(self.pipe_rot)
+
+  def _set(self, pos):
+    return self.pipe_rot[pos]
+
+  def pipeline_load(self, bi, ebase, pos, declare=False):
+    """Registers of logical pipeline position ``pos`` <- block ``bi`` of the group at
+    ``ebase`` (streamed operand rows and per-element right-operand rows)."""
+    w = self.w
+    pl = self.pl
+    reg = self._set(pos)
+    b0, b1 = self._block_rows(bi)
+    nr = b1 - b0
+    if pl.L is not None:
+      L = pl.L
+      total = self.pairs * nr
+      vol = L.arg.element_stride
+      for s_ in range(self._pipe_steps(nr)):
+        if s_ * WAVE + WAVE - 1 < total:
+          idx = f'lane + {s_ * WAVE}'
+        else:
+          idx = f'(lane + {s_ * WAVE} < {total} ? lane + {s_ * WAVE} : {total - 1})'
+        pr = f'(({idx}) / {nr})'
+        kk = f'(({idx}) % {nr})'
+        el = f'({pr} / {self.l_q})'
+        cc = f'({pr} % {self.l_q})'
+        e = f'({ebase} + {el} < numElements ? {ebase} + {el} : 0)'
+        if L.arg.addressing == Addressing.STRIDED:
+          base = f'{L.arg.name} + {e} * {vol}ull + {L.arg.name}_extraOffset'
+        else:
+          base = f'{L.arg.name}[{e}] + {L.arg.name}_extraOffset'
+        goff = (f'({b0 - L.origin[0]} + {kk}) * {L.strides[0]} + '
+                f'({pl.l_cols[0] - L.origin[1]} + {cc}) * {L.strides[1]}')
+        w(f'pf{reg}[{s_}] = ({self.real})({base})[{goff}];')
+    for ti in self.gterms:
+      t = pl.terms[ti]
+      B = t.B
+      kd, nd = t.b_dims
+      a = B.arg
+      eb = f'({ebase} + slot < numElements ? {ebase} + slot : 0)'
+      if a.addressing == Addressing.STRIDED:
+        base = f'{a.name} + {eb} * {a.element_stride}ull + {a.name}_extraOffset'
+      else:
+        base = f'{a.name}[{eb}] + {a.name}_extraOffset'
+      for j, k in enumerate(range(b0, b1)):
+        off = (f'({k - B.origin[kd]}) * {B.strides[kd]} + (col + {pl.cols[0] - B.origin[nd]}) '
+               f'* {B.strides[nd]}')
+        w(f'pg{reg}_{ti}[{j}] = ({self.real})({base})[{off}];')
+    self.pipe_block[pos] = bi
+
+  def pipeline_write(self, bi):
+    """The oldest pipeline position holds block ``bi``: put its streamed rows in the ring and
+    expose its right-operand rows to the compute of that block."""
+    w = self.w
+    pl = self.pl
+    reg = self._set(0)
+    b0, b1 = self._block_rows(bi)
+    nr = b1 - b0
+    if pl.L is not None:
+      total = self.pairs * nr
+      slot_ = bi % 2
+      for s_ in range(self._pipe_steps(nr)):
+        if s_ * WAVE + WAVE - 1 < total:
+          idx = f'lane + {s_ * WAVE}'
+        else:
+          idx = f'(lane + {s_ * WAVE} < {total} ? lane + {s_ * WAVE} : {total - 1})'
+        w(f'ring[{slot_ * self.slot_words} + (({idx}) / {nr}) * {self.ps} + (({idx}) % {nr})] = '
+          f'pf{reg}[{s_}];')
+      w('__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");')
+      w('__builtin_amdgcn_wave_barrier();')
+      w('__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");')
+    # right-operand rows of block bi, as plain values for compute_block
+    vals = {}
+    for ti in self.gterms:
+      for j, k in enumerate(range(b0, b1)):
+        v = self.fresh('gr')
+        w(f'const {self.real} {v} = pg{reg}_{ti}[{j}];')
+        vals[(ti, k)] = v
+    self.pipe_global = [vals]
+
+  def pipeline_realign(self):
+    """Back-edge of the element-group loop: the pending blocks must sit in the register sets
+    the loop entry expects. When the block count is not a multiple of the prefetch distance
+    the rotation has drifted, and the pending sets are copied back."""
+    w = self.w
+    entry = self.pipe_rot_entry
+    now = self.pipe_rot
+    moves = [(now[p], entry[p]) for p in range(self.D - 1) if now[p] != entry[p]]
+    if not moves:
+      return
+    steps = self._pipe_steps(self.KB) if self.pl.L is not None else 0
+    w('{')
+    w._depth += 1
+    for src, dst in moves:
+      if self.pl.L is not None:
+        w(f'{self.real} tpf{src}[{steps}];')
+        for s_ in range(steps):
+          w(f'tpf{src}[{s_}] = pf{src}[{s_}];')
+      for ti in self.gterms:
+        w(f'{self.real} tpg{src}_{ti}[{self.KB}];')
+        for j in range(self.KB):
+          w(f'tpg{src}_{ti}[{j}] = pg{src}_{ti}[{j}];')
+    for src, dst in moves:
+      if self.pl.L is not None:
+        for s_ in range(steps):
+          w(f'pf{dst}[{s_}] = tpf{src}[{s_}];')
+      for ti in self.gterms:
+        for j in range(self.KB):
+          w(f'pg{dst}_{ti}[{j}] = tpg{src}_{ti}[{j}];')
+    w._depth -= 1
+    w('}')
+    self.pipe_rot = list(entry)
+
+  def pipeline_shift(self):
+    """Logical positions move down by one; the freed register set becomes the newest."""
+    self.pipe_rot = self.pipe_rot[1:] + self.pipe_rot[:1]
+    self.pipe_block = self.pipe_block[1:] + [None]
+
+  # ---- runtime block loop (dense shared operands) ----
+  def loop_blocks(self, acc):
+    """The contraction as a runtime loop over blocks of KB rows. The streamed operand's next
+    block and the next block's rows of per-element right operands are loaded one iteration
+    ahead; the ring slot alternates with the block parity."""
+    w = self.w
+    pl = self.pl
+    real = self.real
+    KB = self.KB
+    nb = (pl.khi - pl.klo) // KB
+    nrows = pl.rows[1] - pl.rows[0]
+    nt = len(pl.terms)
+    fused_terms = [ti for ti, t in enumerate(pl.terms) if t.kind == 'fused']
+    offs = sorted(set(o for ti in fused_terms for _, o in self.slots[ti]))
+    if pl.L is not None:
+      self.loop_load(0, 'pf')
+      self.loop_write('pf', '0')
+    gterms = [ti for ti, t in enumerate(pl.terms) if t.kind == 'global']
+    for ti in gterms:
+      for j in range(KB):
+        w(f'{real} gb{ti}_{j} = {self.global_row_expr(ti, f"{pl.klo + j}")};')
+    w('#pragma unroll 1')
+    with w.block(f'for (int kb = 0; kb < {nb}; ++kb)'):
+      w(f'const int k0 = {pl.klo} + kb * {KB};')
+      w(f'const int rs = (kb & 1) * {self.slot_words if pl.L is not None else 0};')
+      if pl.L is not None:
+        self.loop_load(f'(kb + 1 < {nb} ? kb + 1 : kb)', 'nx')
+      for ti in gterms:
+        for j in range(KB):
+          w(f'const {real} cur{ti}_{j} = gb{ti}_{j};')
+        for j in range(KB):
+          w(f'gb{ti}_{j} = {self.global_row_expr(ti, f"(kb + 1 < {nb} ? k0 + {KB + j} : k0 + {j})")};')
+      w(f'asm volatile("" : {", ".join(f"{chr(34)}+v{chr(34)}({o})" for o in ["lane15"] + offs)});')
+      for j in range(KB):
+        rv = {}
+        if fused_terms:
+          qv = {}
+          for o in offs:
+            v = self.fresh('q')
+            w(f'const {real} {v} = ring[rs + {o} + {j}];')
+            qv[o] = v
+        for ti in range(nt):
+          t = pl.terms[ti]
+          if t.kind == 'fused':
+            slots = self.slots[ti]
+            v = self.fresh('r')
+            w(f'{real} {v} = {qv[slots[0][1]]} * {slots[0][0]}_v;')
+            for sv, o in slots[1:]:
+              w(f'{v} = __builtin_fma({qv[o]}, {sv}_v, {v});')
+            rv[ti] = v
+          else:
+            rv[ti] = f'cur{ti}_{j}'
+        for ti in range(nt):
+          for c in range(self.rows_pad // CHUNK):
+            cv = self.fresh('ch')
+            w(f'const {real} {cv} = {self.loop_chunk(ti, f"k0 + {j}", c)};')
+            for jj in range(CHUNK):
+              i = c * CHUNK + jj
+              if i >= nrows:
+                break
+              self.fma(f'{acc}[{i}]', cv, jj, rv[ti])
+      if pl.L is not None:
+        self.loop_write('nx', '((kb + 1) & 1)')
+
+  def loop_chunk(self, ti, kexpr, c):
+    pl = self.pl
+    t = pl.terms[ti]
+    if self.u_source == 'lds':
+      nt = len(pl.terms)
+      return (f'ubuf[((({kexpr}) - {pl.klo}) * {nt} + {ti}) * {self.rows_pad} + {c * CHUNK} + '
+              f'lane15]')
+    U = t.U
+    nd, kd = t.u_dims
+    nrows = pl.rows[1] - pl.rows[0]
+    last = nrows - 1
+    # rows past the window repeat the last row (loaded, never used)
+    row = f'({c * CHUNK} + lane15 < {nrows} ? {c * CHUNK} + lane15 : {last})'
+    return (f'({self.real}){U.arg.name}[({row} + {pl.rows[0] - U.origin[nd]}) * {U.strides[nd]} + '
+            f'(({kexpr}) - {U.origin[kd]}) * {U.strides[kd]}]')
+
+  def global_row_expr(self, ti, kexpr):
+    pl = self.pl
+    B = pl.terms[ti].B
+    kd, nd = pl.terms[ti].b_dims
+    return (f'({self.real})g_{B.arg.name}[(({kexpr}) - {B.origin[kd]}) * {B.strides[kd]} + '
+            f'(col + {pl.cols[0] - B.origin[nd]}) * {B.strides[nd]}]')
+
+  def loop_load(self, kbexpr, tag):
+    """Registers ``<tag>_<s>`` <- rows [kb*KB, kb*KB+KB) of L for the wavefront's pairs."""
+    w = self.w
+    pl = self.pl
+    L = pl.L
+    KB = self.KB
+    total = self.pairs * KB
+    steps = (total + WAVE - 1) // WAVE
+    vol = L.arg.element_stride
+    self.loop_names = getattr(self, 'loop_names', {})
+    names = []
+    for s_ in range(steps):
+      v = f'{tag}_{s_}'
+      if s_ * WAVE + WAVE - 1 < total:
+        idx = f'lane + {s_ * WAVE}'
+      else:
+        idx = f'(lane + {s_ * WAVE} < {total} ? lane + {s_ * WAVE} : {total - 1})'
+      pr = f'(({idx}) / {KB})'
+      kk = f'(({idx}) % {KB})'
+      el = f'({pr} / {self.l_q})'
+      cc = f'({pr} % {self.l_q})'
+      e = f'(e0 + {el} < numElements ? e0 + {el} : 0)'
+      if L.arg.addressing == Addressing.STRIDED:
+        base = f'{L.arg.name} + {e} * {vol}ull + {L.arg.name}_extraOffset'
+      else:
+        base = f'{L.arg.name}[{e}] + {L.arg.name}_extraOffset'
+      goff = (f'({pl.klo - L.origin[0]} + ({kbexpr}) * {KB} + {kk}) * {L.strides[0]} + '
+              f'({pl.l_cols[0] - L.origin[1]} + {cc}) * {L.strides[1]}')
+      if tag not in self.loop_names:
+        w(f'{self.real} {v};')
+      w(f'{v} = ({self.real})({base})[{goff}];')
+      names.append((v, pr, kk))
+    self.loop_names[tag] = names
+
+  def loop_write(self, tag, slot_expr):
+    w = self.w
+    for v, pr, kk in self.loop_names[tag]:
+      w(f'ring[({slot_expr}) * {self.slot_words} + {pr} * {self.ps} + {kk}] = {v};')
+    w('__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");')
+    w('__builtin_amdgcn_wave_barrier();')
+    w('__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");')
 
   # ---- ring of the streamed operand ----
   def _block_rows(self, bi):

@@ -71,9 +71,15 @@ class Schedule:
   # streaming schedule (persistent workgroups, shared operands in LDS, DPP-broadcast FMAs) when
   # the chain has its form; its own workgroup size and contraction block
   stream: bool = True
-  stream_waves: int = 4
-  stream_k_block: int = 8
+  stream_waves: int = 8
+  stream_k_block: int = 4
   stream_min_waves: int = 2
+  # runtime loop over contraction blocks: 'auto' (dense shared operands with many entries),
+  # 'always', 'never'; where the shared entries are read from: 'lds', 'global', 'auto'
+  stream_loop: str = 'auto'
+  stream_u_source: str = 'auto'
+  # contraction blocks of the streamed operand loaded ahead of use (across element groups)
+  stream_prefetch: int = 2
   # how the DPP broadcast is issued: 'asm' (one v_fmac_*_dpp) or 'builtin' (v_mov_*_dpp + fma)
   dpp: str = 'asm'
 
