@@ -438,6 +438,9 @@ class _Stream:
       return
     nb = len(self.blocks)
     D = self.D
+    self._nxt_ptr_done = False
+    if self.own_columns():
+      w(f'const {pl.L.arg.dtype}* __restrict__ lp_cur = {self.lane_pointer("e0")};')
     for bi, b0 in enumerate(self.blocks):
       w(f'// block [{b0}, {min(b0 + self.KB, pl.khi)})')
       # issue the loads of the block D ahead (this group's, or the next group's)
@@ -473,8 +476,11 @@ class _Stream:
     self.pipe_names = []
     for j in range(D):
       if pl.L is not None:
-        w(f'{self.real} pf{j}[{self._pipe_steps(nrmax)}];')
+        size = nrmax if self.own_columns() else self._pipe_steps(nrmax)
+        w(f'{self.real} pf{j}[{size}];')
       self.pipe_names.append(f'pf{j}')
+    if self.own_columns():
+      w(f'const int lane_ro = (slot * {self.l_q} + col) * {self.ps};')
     self.gterms = [ti for ti, t in enumerate(pl.terms) if t.kind == 'global']
     for j in range(D):
       for ti in self.gterms:
@@ -486,6 +492,8 @@ class _Stream:
     w._depth += 1
     w('const size_t e0 = efirst;')
     w('const size_t e0n = efirst + estep;')
+    if self.own_columns():
+      w(f'const {pl.L.arg.dtype}* __restrict__ lp_cur = {self.lane_pointer("e0")};')
     for j in range(D):
       if j < len(self.blocks):
         self.pipeline_load(j, 'e0', j, declare=False)
@@ -498,6 +506,20 @@ class _Stream:
   def _set(self, pos):
     return self.pipe_rot[pos]
 
+  def own_columns(self):
+    """The streamed operand has as many columns as the output: lane (slot, col) loads the
+    rows of its own element's column, so addresses are one per-lane pointer plus constants."""
+    return self.pl.L is not None and self.l_q == self.q and \
+        self.pl.L.arg.addressing == Addressing.STRIDED and self.sched.stream_own_columns
+
+  def lane_pointer(self, ebase):
+    """Expression of the lane's pointer into its element's column of L for group ``ebase``."""
+    L = self.pl.L
+    a = L.arg
+    e = f'({ebase} + slot < numElements ? {ebase} + slot : 0)'
+    return (f'({a.name} + {e} * {a.element_stride}ull + {a.name}_extraOffset + '
+            f'(col + {self.pl.l_cols[0] - L.origin[1]}) * {L.strides[1]})')
+
   def pipeline_load(self, bi, ebase, pos, declare=False):
     """Registers of logical pipeline position ``pos`` <- block ``bi`` of the group at
     ``ebase`` (streamed operand rows and per-element right-operand rows)."""
@@ -506,7 +528,15 @@ class _Stream:
     reg = self._set(pos)
     b0, b1 = self._block_rows(bi)
     nr = b1 - b0
-    if pl.L is not None:
+    if pl.L is not None and self.own_columns():
+      L = pl.L
+      ptr = 'lp_cur' if ebase == 'e0' else 'lp_nxt'
+      if ebase != 'e0' and not getattr(self, '_nxt_ptr_done', False):
+        w(f'const {L.arg.dtype}* __restrict__ lp_nxt = {self.lane_pointer("e0n")};')
+        self._nxt_ptr_done = True
+      for kk in range(nr):
+        w(f'pf{reg}[{kk}] = ({self.real}){ptr}[{(b0 + kk - L.origin[0]) * L.strides[0]}];')
+    elif pl.L is not None:
       L = pl.L
       total = self.pairs * nr
       vol = L.arg.element_stride
@@ -551,7 +581,14 @@ class _Stream:
     reg = self._set(0)
     b0, b1 = self._block_rows(bi)
     nr = b1 - b0
-    if pl.L is not None:
+    if pl.L is not None and self.own_columns():
+      slot_ = bi % 2
+      for kk in range(nr):
+        w(f'ring[{slot_ * self.slot_words} + lane_ro + {kk}] = pf{reg}[{kk}];')
+      w('__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");')
+      w('__builtin_amdgcn_wave_barrier();')
+      w('__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");')
+    elif pl.L is not None:
       total = self.pairs * nr
       slot_ = bi % 2
       for s_ in range(self._pipe_steps(nr)):

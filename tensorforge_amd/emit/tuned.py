@@ -80,6 +80,8 @@ class Schedule:
   stream_u_source: str = 'auto'
   # contraction blocks of the streamed operand loaded ahead of use (across element groups)
   stream_prefetch: int = 2
+  # lane (element, column) streams its own column of the per-element operand
+  stream_own_columns: bool = True
   # how the DPP broadcast is issued: 'asm' (one v_fmac_*_dpp) or 'builtin' (v_mov_*_dpp + fma)
   dpp: str = 'asm'
 
