@@ -102,6 +102,25 @@ def copy_bandwidth(device, steps=20):
   return 2 * n * 8 / t / 1e9
 
 
+def profiled_traffic(kernel_name, schedule):
+  """HBM bytes per launch of ``kernel_name`` from the committed rocprofv3 counter summary
+  (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor the same counter reads for
+  the copy kernel), when that summary was taken with the same schedule; else None."""
+  path = os.path.join(ROOT, 'profiles', f'{kernel_name}_hbm_traffic.md')
+  if not os.path.exists(path):
+    return None
+  fields = {}
+  with open(path, encoding='utf-8') as f:
+    for line in f:
+      if ':' in line and line.startswith('- '):
+        key, _, val = line[2:].partition(':')
+        fields[key.strip()] = val.strip().strip('`')
+  if fields.get('schedule') != json.dumps(schedule, sort_keys=True) or \
+      'traffic_bytes_per_launch' not in fields:
+    return None
+  return int(fields['traffic_bytes_per_launch'])
+
+
 def cpu_baseline(entry, bufs, gpu_out, budget_s):
   """The OpenMP restatement of the reference's generated kernel (oracle/omp_chain.cpp, team
   form and flat form) on the first elements of the same inputs, sized to ~budget_s of CPU work;
@@ -255,7 +274,9 @@ def main():
                            '(24 nnz), FP64', 'elements': n, 'emitter': 'tuned',
                'schedule': emitted.schedule},
     'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBPS, 'unit': 'GB/s',
-                 'frac': achieved / HBM_PEAK_GBPS, 'traffic': None,
+                 'frac': achieved / HBM_PEAK_GBPS,
+                 'traffic': profiled_traffic(f'{entry.name}_{n}', emitted.schedule),
+                 'algorithmic_bytes_per_launch': bytes_per_launch,
                  'bytes_per_element': emitted.bytes_per_element,
                  'measured_copy_GBps': measured_bw,
                  'frac_of_measured_copy': (achieved / measured_bw) if measured_bw else None,
