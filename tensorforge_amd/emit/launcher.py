@@ -32,6 +32,13 @@ class Emitted:
   shared_bytes: int = 0
   grid_override: str = ''       # C expression for the grid size, '' -> ceil(N / per block)
   extra_launch_setup: str = ''
+  # a second kernel of the same chain, launched when ``condition`` (a C expression over the
+  # launcher's parameters) is false
+  fallback: 'Emitted' = None
+  condition: str = ''
+
+  def full_source(self):
+    return self.kernel_src + (self.fallback.full_source() if self.fallback else '')
 
 
 class Writer:
@@ -103,35 +110,45 @@ def kernel_args(program):
 def launcher_source(generator, emitted: Emitted) -> str:
   w = Writer()
   proto = generator._generate_launcher_proto(with_defaults=False)
-  program = generator.get_program()
-  kname = emitted.kernel_name
-  bx, by, bz = emitted.block
   with w.block(f'extern "C" {proto}'):
     w(f'if ({GeneralLexicon.NUM_ELEMENTS} == 0) return;')
     w(f'hipStream_t stream = ({GeneralLexicon.STREAM_PTR_STR} != nullptr) ? '
       f'static_cast<hipStream_t>({GeneralLexicon.STREAM_PTR_STR}) : 0;')
-    w(f'dim3 block({bx}, {by}, {bz});')
-    per_block = emitted.elements_per_block
-    if emitted.extra_launch_setup:
-      w.lines(emitted.extra_launch_setup)
-    if emitted.grid_override:
-      w(f'dim3 grid({emitted.grid_override}, 1, 1);')
+    if emitted.fallback is not None:
+      with w.block(f'if ({emitted.condition})'):
+        _launch_block(w, generator, emitted)
+      with w.block('else'):
+        _launch_block(w, generator, emitted.fallback)
     else:
-      w(f'dim3 grid(static_cast<unsigned>(({GeneralLexicon.NUM_ELEMENTS} + {per_block} - 1) / '
-        f'{per_block}), 1, 1);')
-    if emitted.lds_bytes > 64 * 1024:
-      with w.block('static bool shmemsizeset = false; if (!shmemsizeset)'):
-        w(f'hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&{kname}), '
-          f'hipFuncAttributeMaxDynamicSharedMemorySize, {emitted.lds_bytes});')
-        w(f'if (err != hipSuccess) {{ tensorforge_record_error(static_cast<int>(err), '
-          f'"launcher_{generator.get_base_name()}"); return; }}')
-        w('shmemsizeset = true;')
-    args = ', '.join(kernel_args(program))
-    w(f'hipLaunchKernelGGL({kname}, grid, block, {emitted.lds_bytes}, stream, {args});')
-    w('hipError_t err = hipGetLastError();')
-    w(f'if (err != hipSuccess) tensorforge_record_error(static_cast<int>(err), '
-      f'"launcher_{generator.get_base_name()}");')
+      _launch_block(w, generator, emitted)
   return w.src()
+
+
+def _launch_block(w, generator, emitted):
+  program = generator.get_program()
+  kname = emitted.kernel_name
+  bx, by, bz = emitted.block
+  w(f'dim3 block({bx}, {by}, {bz});')
+  per_block = emitted.elements_per_block
+  if emitted.extra_launch_setup:
+    w.lines(emitted.extra_launch_setup)
+  if emitted.grid_override:
+    w(f'dim3 grid({emitted.grid_override}, 1, 1);')
+  else:
+    w(f'dim3 grid(static_cast<unsigned>(({GeneralLexicon.NUM_ELEMENTS} + {per_block} - 1) / '
+      f'{per_block}), 1, 1);')
+  if emitted.lds_bytes > 64 * 1024:
+    with w.block('static bool shmemsizeset = false; if (!shmemsizeset)'):
+      w(f'hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&{kname}), '
+        f'hipFuncAttributeMaxDynamicSharedMemorySize, {emitted.lds_bytes});')
+      w(f'if (err != hipSuccess) {{ tensorforge_record_error(static_cast<int>(err), '
+        f'"launcher_{generator.get_base_name()}"); return; }}')
+      w('shmemsizeset = true;')
+  args = ', '.join(kernel_args(program))
+  w(f'hipLaunchKernelGGL({kname}, grid, block, {emitted.lds_bytes}, stream, {args});')
+  w('hipError_t err = hipGetLastError();')
+  w(f'if (err != hipSuccess) tensorforge_record_error(static_cast<int>(err), '
+    f'"launcher_{generator.get_base_name()}");')
 
 
 def translation_unit(generators) -> str:

@@ -146,14 +146,16 @@ def plan(program: ir.ChainProgram):
 
 
 class _Stream:
-  def __init__(self, program, context, base_name, schedule, pl: Plan):
+  def __init__(self, program, context, base_name, schedule, pl: Plan, allow_dma=True,
+               suffix=''):
+    self.allow_dma = allow_dma
     self.prog = program
     self.ctx = context
     self.fp = program.fp_type
     self.real = FloatingPointType.as_str(self.fp)
     self.sched = schedule
     self.pl = pl
-    self.kname = f'kernel_{base_name}'
+    self.kname = f'kernel_{base_name}{suffix}'
     self.w = Writer()
     self.q = pl.q
     self.G = WAVE // pl.q
@@ -176,6 +178,7 @@ class _Stream:
     lo0 = pl.rows[0]
     seq = []
     for t_idx, t in enumerate(pl.terms):
+      t.entries = []
       U = t.U
       nd, kd = t.u_dims
       if U.is_sparse():
@@ -219,6 +222,11 @@ class _Stream:
     pl = self.pl
     self.order_entries()
     self.loop = self._loop_mode()
+    if self.loop and self.KB != self.sched.stream_loop_k_block:
+      # the runtime block loop keeps one block of every term in flight: smaller blocks
+      self.KB = max(2, self.sched.stream_loop_k_block)
+      self.order_entries()
+      self.loop = self._loop_mode()
     self.rows_pad = (pl.rows[1] - pl.rows[0] + CHUNK - 1) // CHUNK * CHUNK
     if self.loop:
       # shared entries laid out column by column, each (column, term) padded to whole chunks
@@ -233,12 +241,23 @@ class _Stream:
     W, G, q, KB = self.W, self.G, self.q, self.KB
     threads = WAVE * W
     # ring of the streamed operand: per wavefront 2 slots x pairs x (KB + 1) rows
+    self.dma = False
     if pl.L is not None:
       self.l_q = pl.l_cols[1] - pl.l_cols[0]
       self.pairs = G * self.l_q
       self.ps = KB + 1
       self.slot_words = self.pairs * self.ps
       ring = 2 * self.slot_words
+      if self.allow_dma and self.dma_possible():
+        # rows of the streamed operand go straight from memory to LDS: each load instruction
+        # moves 16 bytes (rpl rows of its column) per lane, lane i's rows land at i * rpl of
+        # the instruction's region; ring of NS slots of KB rows x 64 lanes
+        self.dma = True
+        self.rpl = 16 // self.fp.size()
+        self.ns = max(2, min(self.sched.stream_dma_slots, len(self.blocks) + 1))
+        self.ps = self.rpl
+        self.slot_words = KB * WAVE
+        ring = self.ns * self.slot_words
     else:
       ring = 0
     self.ring_words = ring
@@ -297,7 +316,9 @@ class _Stream:
     w('const size_t rounds = (ngroups + stride - 1) / stride;')
     w(f'const size_t efirst = ((size_t)blockIdx.x * {W} + wave) * {G};')
     w(f'const size_t estep = stride * {G};')
-    if not self.loop:
+    if self.dma:
+      self.dma_prologue()
+    elif not self.loop:
       self.pipeline_prologue()
     with w.block('for (size_t r = 0; r < rounds; ++r)'):
       w('const size_t e0 = efirst + r * estep;')
@@ -307,18 +328,29 @@ class _Stream:
         '(flags == nullptr || flags[batchId] != 0u);')
       w('const size_t eSafe = active ? batchId : 0;')
       self.group_body()
+    if self.dma:
+      # loads issued for the group after the last one (clamped to element 0) must land first
+      w('asm volatile("s_waitcnt vmcnt(0)" ::: "memory");')
     w._depth -= 1
     w('}')
     traffic = element_traffic(self.prog)
     sched = {'kind': 'stream', 'waves_per_block': W, 'k_block': KB, 'elements_per_wave': G,
              'columns': q, 'shared_entries': self.nu, 'lds_bytes': lds_bytes,
              'dpp': self.sched.dpp}
+    if self.dma:
+      sched['lds_dma_slots'] = self.ns
+    fallback = None
+    if self.dma:
+      fallback = _Stream(self.prog, self.ctx, self.kname[len('kernel_'):], self.sched, self.pl,
+                         allow_dma=False, suffix='_r').emit()
     return Emitted(kernel_name=self.kname, kernel_src=w.src(), block=(threads, 1, 1),
                    elements_per_block=G * W, lds_bytes=lds_bytes, emitter='tuned',
                    schedule=sched, bytes_per_element=traffic.per_element,
                    shared_bytes=traffic.shared,
                    grid_override='tf_grid',
-                   extra_launch_setup=self.grid_setup(lds_bytes, threads))
+                   extra_launch_setup=self.grid_setup(lds_bytes, threads),
+                   fallback=fallback,
+                   condition=self.dma_condition() if self.dma else '')
 
   def grid_setup(self, lds_bytes, threads):
     G, W = self.G, self.W
@@ -437,6 +469,13 @@ class _Stream:
           w(f'g_{pl.dest.name}[{off}] = {acc}[{i}];')
       return
     nb = len(self.blocks)
+    if self.dma:
+      self.dma_blocks(acc)
+      with w.block('if (active)'):
+        for i in range(rows):
+          off = f'{(pl.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
+          w(f'g_{pl.dest.name}[{off}] = {acc}[{i}];')
+      return
     D = self.D
     self._nxt_ptr_done = False
     if self.own_columns():
@@ -450,7 +489,10 @@ class _Stream:
       else:
         self.pipeline_load(tb - nb, 'e0n', D - 1)
       self.global_rows = {bi: self.pipe_global[0]}
-      self.compute_block(bi, b0, acc)
+      if self.sched.stream_lds_lookahead > 0:
+        self.compute_block_pipelined(bi, b0, acc)
+      else:
+        self.compute_block(bi, b0, acc)
       # block bi + 1 (or the next group's first block) into the ring
       nxt = bi + 1 if bi + 1 < nb else 0
       self.pipeline_write(nxt)
@@ -502,6 +544,88 @@ class _Stream:
     self.pipeline_write(0)
     self.pipeline_shift()
     self.pipe_rot_entry = list(self.pipe_rot)
+
+  # ---- direct memory-to-LDS ring (rows of the streamed operand bypass registers) ----
+  def dma_possible(self):
+    """Direct loads need 16-byte aligned row groups: rows contiguous, column and element
+    strides multiples of 16 bytes, blocks made of whole row groups (the base pointer's
+    alignment is checked by the launcher)."""
+    pl = self.pl
+    L = pl.L
+    if not (self.sched.stream_dma and not self.loop and self.own_columns()):
+      return False
+    if any(t.kind == 'global' for t in pl.terms) or len(self.blocks) < 2:
+      return False
+    size = self.fp.size()
+    rpl = 16 // size
+    return (L.strides[0] == 1 and (L.strides[1] * size) % 16 == 0 and
+            (L.arg.element_stride * size) % 16 == 0 and self.KB % rpl == 0 and
+            (pl.khi - pl.klo) % rpl == 0 and (pl.klo - L.origin[0]) % rpl == 0 and
+            ((pl.l_cols[0] - L.origin[1]) * L.strides[1] * size) % 16 == 0)
+
+  def dma_condition(self):
+    a = self.pl.L.arg
+    return f'(reinterpret_cast<uintptr_t>({a.name} + {a.name}_extraOffset) % 16u == 0u)'
+
+  def ring_ref(self, ring_slot, o, kk):
+    if self.dma:
+      r = self.rpl
+      return f'rb[{o} + {(kk // r) * WAVE * r + kk % r}]'
+    return f'ring[{ring_slot * self.slot_words} + {o} + {kk}]'
+
+  def dma_issue(self, bi, ptr, slot_expr):
+    """Loads of block ``bi``'s rows of the lane's column (pointer ``ptr``) into ring slot
+    ``slot_expr``: one instruction per row, lane i's value lands at row base + i."""
+    w = self.w
+    L = self.pl.L
+    b0, b1 = self._block_rows(bi)
+    r = self.rpl
+    for j in range((b1 - b0) // r):
+      goff = b0 + j * r - L.origin[0]
+      w(f'__builtin_amdgcn_global_load_lds((const void*)({ptr} + {goff}), '
+        f'(tf_lds_t*)(ring + ({slot_expr}) * {self.slot_words} + {j * WAVE * r}), 16, 0, 0);')
+
+  def dma_prologue(self):
+    w = self.w
+    nb = len(self.blocks)
+    self.D = self.ns - 1
+    w('typedef __attribute__((address_space(3))) void tf_lds_t;')
+    with w.block(''):
+      w(f'const {self.pl.L.arg.dtype}* __restrict__ lp0 = {self.lane_pointer("efirst")};')
+      for j in range(self.D):
+        self.dma_issue(j, 'lp0', str(j))
+
+  def dma_blocks(self, acc):
+    """Blocks of one element group: wait for block bi's rows, start the loads of the block
+    D ahead (this group's or the next one's) into the slot freed by block bi - 1, compute."""
+    w = self.w
+    pl = self.pl
+    nb = len(self.blocks)
+    D, NS, KB = self.D, self.ns, self.KB
+    L = pl.L
+    w(f'const {L.arg.dtype}* __restrict__ lp_cur = {self.lane_pointer("e0")};')
+    w(f'const {L.arg.dtype}* __restrict__ lp_nxt = {self.lane_pointer("e0n")};')
+    w(f'const unsigned gs = (unsigned)((r * {nb}) % {NS});')
+    # at most the loads of the D - 1 younger blocks may still be in flight
+    pending = min(63, (D - 1) * (KB // self.rpl))
+    for bi, b0 in enumerate(self.blocks):
+      w(f'// block [{b0}, {min(b0 + KB, pl.khi)})')
+      w(f'asm volatile("s_waitcnt vmcnt({pending})" ::: "memory");')
+      w('{')
+      w._depth += 1
+      w(f'{self.real}* __restrict__ rb = ring + ((gs + {bi}u) % {NS}u) * {self.slot_words};')
+      tb = bi + D
+      if tb < nb:
+        self.dma_issue(tb, 'lp_cur', f'(gs + {tb}u) % {NS}u')
+      else:
+        self.dma_issue(tb - nb, 'lp_nxt', f'(gs + {tb}u) % {NS}u')
+      if self.sched.stream_lds_lookahead > 0:
+        self.compute_block_pipelined(bi, b0, acc)
+      else:
+        self.compute_block(bi, b0, acc)
+      w._depth -= 1
+      w('}')
+      w('__builtin_amdgcn_sched_barrier(0);')
 
   def _set(self, pos):
     return self.pipe_rot[pos]
@@ -881,7 +1005,7 @@ class _Stream:
         qv = {}
         for o in offs:
           v = self.fresh('q')
-          w(f'const {real} {v} = ring[{ring_slot * self.slot_words} + {o} + {k - b0}];')
+          w(f'const {real} {v} = {self.ring_ref(ring_slot, o, k - b0)};')
           qv[o] = v
       for ti in terms_here:
         t = pl.terms[ti]
@@ -903,6 +1027,101 @@ class _Stream:
         self.fma(f'{acc}[{i}]', chunks[c], j, rvals[ti])
         if i not in touched:
           touched.append(i)
+    for n0 in range(0, len(touched), 8):
+      ops = ', '.join(f'"+v"({acc}[{i}])' for i in touched[n0:n0 + 8])
+      w(f'asm volatile("" : {ops});')
+
+  def compute_block_pipelined(self, bi, b0, acc):
+    """compute_block with LDS reads issued ahead of their use: the shared-operand chunks
+    ``stream_lds_lookahead`` chunks ahead, the streamed rows of step k+1 (and the fused
+    temporary rows they give) during step k. Scheduling barriers keep the order, so the
+    latency of each LDS read is covered by the FMAs issued in between."""
+    w = self.w
+    pl = self.pl
+    real = self.real
+    b1 = min(b0 + self.KB, pl.khi)
+    ring_slot = bi % 2
+    ahead = self.sched.stream_lds_lookahead
+    block_seq = [(n, s) for n, s in enumerate(self.seq) if s[0] == b0]
+    fused_terms = [ti for ti, t in enumerate(pl.terms) if t.kind == 'fused']
+    offs = sorted(set(o for ti in fused_terms for _, o in self.slots[ti]))
+    w(f'asm volatile("" : {", ".join(f"{chr(34)}+v{chr(34)}({o})" for o in ["lane15"] + offs)});')
+    steps = []
+    for k in range(b0, b1):
+      ents = [(n, s) for n, s in block_seq if s[2][1] == k]
+      if ents:
+        steps.append((k, ents))
+    if not steps:
+      return
+    order = []
+    for _, ents in steps:
+      for n, _ in ents:
+        if n // CHUNK not in order:
+          order.append(n // CHUNK)
+    cvar = {}
+    issued = [0]
+
+    def issue(upto):
+      while issued[0] < min(upto, len(order)):
+        c = order[issued[0]]
+        v = self.fresh('ch')
+        w(f'const {real} {v} = ubuf[{c * CHUNK} + lane15];')
+        cvar[c] = v
+        issued[0] += 1
+
+    def q_reads(k, ents):
+      terms = set(s[1] for _, s in ents)
+      if not any(pl.terms[ti].kind == 'fused' for ti in terms):
+        return {}
+      qv = {}
+      for o in offs:
+        v = self.fresh('q')
+        w(f'const {real} {v} = {self.ring_ref(ring_slot, o, k - b0)};')
+        qv[o] = v
+      return qv
+
+    def t_rows(k, ents, qv):
+      rv = {}
+      for ti in sorted(set(s[1] for _, s in ents)):
+        t = pl.terms[ti]
+        if t.kind == 'fused':
+          slots = self.slots[ti]
+          v = self.fresh('r')
+          w(f'{real} {v} = {qv[slots[0][1]]} * {slots[0][0]}_v;')
+          for sv, o in slots[1:]:
+            w(f'{v} = __builtin_fma({qv[o]}, {sv}_v, {v});')
+          rv[ti] = v
+        else:
+          rv[ti] = self.global_rows[bi][(ti, k)]
+      return rv
+
+    barrier = '__builtin_amdgcn_sched_barrier(0);'
+    issue(ahead)
+    rvals = t_rows(steps[0][0], steps[0][1], q_reads(steps[0][0], steps[0][1]))
+    touched = []
+    for si, (k, ents) in enumerate(steps):
+      nxt = steps[si + 1] if si + 1 < len(steps) else None
+      w(barrier)
+      nq = q_reads(nxt[0], nxt[1]) if nxt else {}
+      groups = []
+      for n, (_, ti, (i, kk, pos)) in ents:
+        c, j = divmod(n, CHUNK)
+        if not groups or groups[-1][0] != c:
+          groups.append((c, []))
+        groups[-1][1].append((ti, i, j))
+      nrv = None
+      for gi, (c, items) in enumerate(groups):
+        issue(order.index(c) + 1 + ahead)
+        w(barrier)
+        if gi == min(1, len(groups) - 1) and nxt is not None:
+          nrv = t_rows(nxt[0], nxt[1], nq)
+        for ti, i, j in items:
+          self.fma(f'{acc}[{i}]', cvar[c], j, rvals[ti])
+          if i not in touched:
+            touched.append(i)
+      if nxt is not None:
+        rvals = nrv
+    w(barrier)
     for n0 in range(0, len(touched), 8):
       ops = ', '.join(f'"+v"({acc}[{i}])' for i in touched[n0:n0 + 8])
       w(f'asm volatile("" : {ops});')

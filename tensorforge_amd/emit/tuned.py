@@ -72,16 +72,24 @@ class Schedule:
   # the chain has its form; its own workgroup size and contraction block
   stream: bool = True
   stream_waves: int = 8
-  stream_k_block: int = 4
+  stream_k_block: int = 8
+  # contraction block of the runtime block loop (dense shared operands with many entries)
+  stream_loop_k_block: int = 4
   stream_min_waves: int = 2
   # runtime loop over contraction blocks: 'auto' (dense shared operands with many entries),
   # 'always', 'never'; where the shared entries are read from: 'lds', 'global', 'auto'
   stream_loop: str = 'auto'
   stream_u_source: str = 'auto'
   # contraction blocks of the streamed operand loaded ahead of use (across element groups)
-  stream_prefetch: int = 2
+  stream_prefetch: int = 1
   # lane (element, column) streams its own column of the per-element operand
   stream_own_columns: bool = True
+  # shared-operand chunks loaded ahead of use (0: loads placed by the compiler)
+  stream_lds_lookahead: int = 2
+  # rows of the streamed operand loaded straight into LDS (global_load_lds), ring of this many
+  # blocks (prefetch distance = slots - 1)
+  stream_dma: bool = False
+  stream_dma_slots: int = 3
   # how the DPP broadcast is issued: 'asm' (one v_fmac_*_dpp) or 'builtin' (v_mov_*_dpp + fma)
   dpp: str = 'asm'
 
@@ -830,5 +838,11 @@ def emit(program, context, base_name, schedule=None) -> Emitted:
     if pl is not None:
       sched = Schedule(**{**asdict(schedule), 'waves_per_block': schedule.stream_waves,
                           'k_block': schedule.stream_k_block})
-      return stream.emit(program, context, base_name, sched, pl)
+      emitted = stream.emit(program, context, base_name, sched, pl)
+      if emitted.lds_bytes <= LDS_LIMIT:
+        return emitted
   return _Emitter(program, context, base_name, schedule).emit()
+
+
+# LDS per workgroup on gfx950
+LDS_LIMIT = 160 * 1024

@@ -162,7 +162,7 @@ class Generator:
       emitted = plain.emit(self._program, self._context, self._base_kernel_name,
                            self._num_threads, self._thread_block_policy_type)
     self._emitted = emitted
-    self._kernel = emitted.kernel_src
+    self._kernel = emitted.full_source()
     self._launcher = launcher.launcher_source(self, emitted)
     self._header = f'{self._generate_launcher_proto(with_defaults=True)};\n'
 

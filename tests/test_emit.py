@@ -19,9 +19,11 @@ def test_stream_schedule_one_fma_per_stored_entry():
   em = g.get_emitted()
   assert em.schedule['kind'] == 'stream'
   nnz = sum(int(m.sum()) for m in w.meta['masks'])
-  src = g.get_kernel()
-  assert src.count('v_fmac_f64_dpp') == nnz
+  kernels = [em] + ([em.fallback] if em.fallback else [])
+  for k in kernels:
+    assert k.kernel_src.count('v_fmac_f64_dpp') == nnz
   assert em.schedule['shared_entries'] == nnz
+  src = g.get_kernel()
   # no scalar-memory reads of the shared matrices inside the loop: they come from LDS chunks
   assert 'ubuf[' in src
 
@@ -29,7 +31,7 @@ def test_stream_schedule_one_fma_per_stored_entry():
 def test_stream_schedule_recognises_the_single_gemm():
   g = _gen(W.gemm())
   assert g.get_emitted().schedule['kind'] == 'stream'
-  assert g.get_kernel().count('v_fmac_f64_dpp') == 56 * 56
+  assert g.get_emitted().kernel_src.count('v_fmac_f64_dpp') == 56 * 56
 
 
 def test_stream_plan_rejects_other_shapes():
@@ -53,3 +55,18 @@ def test_plain_schedule_follows_reference_mapping():
   g.generate()
   em = g.get_emitted()
   assert em.block == (32, 8, 1) and em.emitter == 'plain'
+
+
+def test_stream_direct_lds_ring_has_an_unaligned_fallback():
+  """The direct memory-to-LDS ring needs 16-byte aligned rows: the launcher checks the streamed
+  operand's base (pointer + extraOffset) and launches the register-ring kernel otherwise."""
+  g = _gen(W.ader_volume(), stream_dma=True)
+  em = g.get_emitted()
+  assert em.schedule.get('lds_dma_slots', 0) >= 2
+  assert 'global_load_lds' in em.kernel_src and 'global_load_lds' not in em.fallback.kernel_src
+  assert em.fallback.kernel_name == em.kernel_name + '_r'
+  launcher = g.get_launcher()
+  assert '% 16u == 0u' in launcher and em.fallback.kernel_name in launcher
+  # odd row counts cannot use 16-byte row pairs: no direct ring
+  g5 = _gen(W.ader_volume(b=35, sparse_k=False), stream_dma=True, stream_loop='never')
+  assert 'lds_dma_slots' not in g5.get_emitted().schedule
