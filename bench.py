@@ -33,7 +33,7 @@ def parse():
   ap.add_argument('--elements', type=int, default=262144)
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--cpu-seconds', type=float, default=15.0)
-  ap.add_argument('--only', default='', help='comma list of legs (tuned,plain,f32,copy)')
+  ap.add_argument('--only', default='', help='comma list of legs (tuned,plain,f32,copy,mixed)')
   return ap.parse_args()
 
 
@@ -100,6 +100,105 @@ def copy_bandwidth(device, steps=20):
   t = start.elapsed_time(stop) / 1e3 / steps
   del src, dst
   return 2 * n * 8 / t / 1e9
+
+
+def mixed_inputs(group, counts, device, seed=3):
+  """Per-order device buffers of the uneven batch: Q ~ N(0,1), star values U[-2,2], dense
+  K_d U[-1,1], Q' zero."""
+  import torch
+  from tensorforge_amd import workloads as W
+  g = torch.Generator(device=device)
+  g.manual_seed(seed)
+  out = []
+  for e, n in zip(group.members, counts):
+    b, q = e.meta['b'], e.meta['q']
+    bufs = {f'K{d + 1}': torch.rand(b * b, generator=g, device=device, dtype=torch.float64) * 2 - 1
+            for d in range(3)}
+    bufs['Q'] = torch.randn(n * b * q, generator=g, device=device, dtype=torch.float64)
+    nnz_s = W.packed_count(e.tensors['S1'])
+    for d in range(3):
+      sv = torch.zeros(n, q * q, device=device, dtype=torch.float64)
+      sv[:, :nnz_s] = torch.rand(n, nnz_s, generator=g, device=device, dtype=torch.float64) * 4 - 2
+      bufs[f'S{d + 1}'] = sv.reshape(-1)
+    bufs['Qp'] = torch.zeros(n * b * q, device=device, dtype=torch.float64)
+    out.append(bufs)
+  return out
+
+
+def mixed_leg(device, steps, warmup, graph_steps=10):
+  """Uneven batch (orders 2..7, 500,000 elements, dense K_d, FP64): one step as plain
+  per-order launches, tuned per-order launches and one grouped persistent launch; and
+  ``graph_steps`` steps with Q and Q' swapping roles captured as one HIP graph (grouped launches,
+  and tuned per-order launches)."""
+  import torch
+  from tensorforge_amd import catalog
+  from tensorforge_amd import workloads as W
+  from tensorforge_amd.device import launch_args
+  from tensorforge_amd.graph import Graph
+  group, tuned, plain, gk = catalog.load_group('ader_mixed_f64')
+  counts = W.mixed_counts()
+  total = sum(counts)
+  bufs = mixed_inputs(group, counts, device)
+  stream = torch.cuda.Stream()
+
+  def member_args(swap):
+    out = []
+    for e, b, n in zip(group.members, bufs, counts):
+      bb = dict(b, Q=b['Qp'], Qp=b['Q']) if swap else b
+      out.append(launch_args(e.generator, bb, n, stream=stream))
+    return out
+
+  args = [member_args(False), member_args(True)]
+  gargs = [group.grouped.member_args(a) for a in args]
+
+  def timed(fn, n_steps, n_warm):
+    with torch.cuda.stream(stream):
+      for _ in range(n_warm):
+        fn()
+      torch.cuda.synchronize()
+      t0 = torch.cuda.Event(enable_timing=True)
+      t1 = torch.cuda.Event(enable_timing=True)
+      t0.record(stream)
+      for _ in range(n_steps):
+        fn()
+      t1.record(stream)
+      torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / 1e3 / n_steps
+
+  res = {'elements': total, 'counts': dict(zip((str(b) for b in W.MIXED_BASES), counts))}
+  t_plain = timed(lambda: [k(*a) for k, a in zip(plain, args[0])], max(2, steps // 10), 1)
+  t_tuned = timed(lambda: [k(*a) for k, a in zip(tuned, args[0])], steps, warmup)
+  t_group = timed(lambda: gk(*gargs[0]), steps, warmup)
+
+  def seq_grouped():
+    for s in range(graph_steps):
+      gk(*gargs[s % 2])
+
+  def seq_members():
+    for s in range(graph_steps):
+      for k, a in zip(tuned, args[s % 2]):
+        k(*a)
+
+  graphs = {}
+  for name, fn in (('grouped', seq_grouped), ('per_order', seq_members)):
+    gr = Graph(stream)
+    gr.capture(fn)
+    t = timed(gr.launch, max(2, steps // graph_steps), 1) / graph_steps
+    graphs[name] = t
+    gr.close()
+  res.update({
+    'plain_per_order': {'value': total / t_plain, 'ms_per_step': t_plain * 1e3},
+    'tuned_per_order': {'value': total / t_tuned, 'ms_per_step': t_tuned * 1e3},
+    'grouped': {'value': total / t_group, 'ms_per_step': t_group * 1e3},
+    'graph_grouped': {'value': total / graphs['grouped'], 'ms_per_step': graphs['grouped'] * 1e3,
+                      'steps_per_graph': graph_steps},
+    'graph_tuned_per_order': {'value': total / graphs['per_order'],
+                              'ms_per_step': graphs['per_order'] * 1e3,
+                              'steps_per_graph': graph_steps},
+    'grouped_over_plain': t_plain / t_group,
+    'unit': 'elements/s'})
+  del bufs
+  return res
 
 
 def profiled_traffic(kernel_name, schedule):
@@ -194,7 +293,7 @@ def main():
   import torch
   from tensorforge_amd import catalog
   device = torch.device('cuda:0')
-  legs = set(args.only.split(',')) if args.only else {'tuned', 'plain', 'f32', 'copy'}
+  legs = set(args.only.split(',')) if args.only else {'tuned', 'plain', 'f32', 'copy', 'mixed'}
   n = args.elements
   out = {}
 
@@ -249,6 +348,9 @@ def main():
                   'hbm_GBps': (bpe32 * n + k32.generator.get_emitted().shared_bytes) / t32 / 1e9,
                   'max_normwise_rel_err_vs_f64_kernel': err32}
     del b32
+
+  if 'mixed' in legs:
+    out['mixed_orders'] = mixed_leg(device, args.steps, args.warmup)
 
   measured_bw = copy_bandwidth(device) if 'copy' in legs else None
 

@@ -39,6 +39,16 @@ int tensorforge_synch_device(void* stream);
  * benchmarks/bandwidth/global.cpp:30-39 and benchmarks/bandwidth/hip_kernel.cpp:4-13. */
 int tensorforge_copy(double* dst, const double* src, size_t n, void* stream);
 
+/* HIP-graph capture of launcher sequences (time-step loops). New: the reference has no graph
+ * support. begin/end bracket launcher calls issued on `stream` (a non-null hipStream_t; call each
+ * launcher once before capturing it, so its one-time device queries are done); end returns an
+ * instantiated graph in *graph_exec; launch replays it on `stream`; destroy frees it. Each returns
+ * the HIP status. */
+int tensorforge_graph_begin(void* stream);
+int tensorforge_graph_end(void* stream, void** graph_exec);
+int tensorforge_graph_launch(void* graph_exec, void* stream);
+int tensorforge_graph_destroy(void* graph_exec);
+
 #ifdef __cplusplus
 }
 #endif

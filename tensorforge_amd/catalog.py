@@ -84,9 +84,84 @@ def _build_one(name):
 def build_catalog(selection=None, jobs=None):
   rt_build.build_runtime()
   todo = list(selection) if selection else names()
+  groups = [] if selection else list(_GROUPS)
   jobs = jobs or min(8, os.cpu_count() or 1)
   with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
-    return list(pool.map(_build_one, todo))
+    futs = [pool.submit(_build_one, n) for n in todo]
+    futs += [pool.submit(build_group, g, part) for g in groups for part in ('tuned', 'plain')]
+    return [f.result() for f in futs]
+
+
+# --------------------------------------------------------------- grouped launches
+
+@dataclass
+class GroupEntry:
+  """Several chains launched together: ``members`` (tuned, streaming schedule) and the
+  grouped launcher over them, plus the same chains with the plain emitter (built into a
+  separate object: plain and tuned launchers of a chain share the kernel name)."""
+  name: str
+  fp_type: object
+  workloads: list
+  members: List[Entry] = field(default_factory=list)
+  plain: List[Entry] = field(default_factory=list)
+  grouped: object = None
+  meta: dict = field(default_factory=dict)
+
+
+_GROUPS: Dict[str, Callable[[], GroupEntry]] = {}
+
+
+def register_group(name):
+  def deco(fn):
+    _GROUPS[name] = fn
+    return fn
+  return deco
+
+
+def group_names():
+  return list(_GROUPS)
+
+
+def get_group(name) -> GroupEntry:
+  from .grouped import GroupedGenerator
+  g = _GROUPS[name]()
+  g.name = name
+  for w in g.workloads:
+    for emitter, dst in (('tuned', g.members), ('plain', g.plain)):
+      e = _wl_entry(w, emitter)
+      e.name = f'{name}_b{w.meta["b"]}_{emitter}'
+      e.make_generator()
+      e.generator.generate()
+      dst.append(e)
+  g.grouped = GroupedGenerator([e.generator for e in g.members])
+  g.grouped.generate()
+  return g
+
+
+def group_so_path(name, part):
+  return os.path.join(rt_build.LIB_DIR, f'libtf_{name}_{part}.so')
+
+
+def build_group(name, part):
+  g = get_group(name)
+  gens = [e.generator for e in g.members] + [g.grouped] if part == 'tuned' else \
+      [e.generator for e in g.plain]
+  return rt_build.compile_generators(gens, f'tf_{name}_{part}')
+
+
+def load_group(name):
+  """(GroupEntry, tuned member kernels, plain member kernels, grouped kernel)."""
+  g = get_group(name)
+  tuned = rt_loader.load_kernels(group_so_path(name, 'tuned'),
+                                 [e.generator for e in g.members] + [g.grouped])
+  plain = rt_loader.load_kernels(group_so_path(name, 'plain'), [e.generator for e in g.plain])
+  return g, tuned[:-1], plain, tuned[-1]
+
+
+@register_group('ader_mixed_f64')
+def _mixed_f64():
+  return GroupEntry(name='', fp_type=F64, workloads=W.ader_mixed(F64),
+                    meta={'bases': list(W.MIXED_BASES)})
 
 
 # --------------------------------------------------------------- workloads

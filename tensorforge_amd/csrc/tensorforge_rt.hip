@@ -76,3 +76,42 @@ extern "C" int tensorforge_copy(double* dst, const double* src, size_t n, void* 
   if (err != hipSuccess) tensorforge_record_error(static_cast<int>(err), "tensorforge_copy");
   return static_cast<int>(err);
 }
+
+// Time-step sequences as HIP graphs: the caller brackets a sequence of launcher calls on one
+// stream with begin/end, then replays the instantiated graph. Launchers that query the device
+// on first use (occupancy of persistent kernels) must have been called once before capture.
+extern "C" int tensorforge_graph_begin(void* stream) {
+  if (stream == nullptr) return static_cast<int>(hipErrorInvalidValue);
+  hipError_t err = hipStreamBeginCapture(static_cast<hipStream_t>(stream),
+                                         hipStreamCaptureModeThreadLocal);
+  if (err != hipSuccess) tensorforge_record_error(static_cast<int>(err), "tensorforge_graph_begin");
+  return static_cast<int>(err);
+}
+
+extern "C" int tensorforge_graph_end(void* stream, void** graph_exec) {
+  if (stream == nullptr || graph_exec == nullptr) return static_cast<int>(hipErrorInvalidValue);
+  *graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipError_t err = hipStreamEndCapture(static_cast<hipStream_t>(stream), &graph);
+  if (err == hipSuccess) {
+    hipGraphExec_t exec = nullptr;
+    err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (err == hipSuccess) *graph_exec = exec;
+  }
+  if (graph != nullptr) (void)hipGraphDestroy(graph);
+  if (err != hipSuccess) tensorforge_record_error(static_cast<int>(err), "tensorforge_graph_end");
+  return static_cast<int>(err);
+}
+
+extern "C" int tensorforge_graph_launch(void* graph_exec, void* stream) {
+  if (graph_exec == nullptr) return static_cast<int>(hipErrorInvalidValue);
+  hipError_t err = hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec),
+                                  static_cast<hipStream_t>(stream));
+  if (err != hipSuccess) tensorforge_record_error(static_cast<int>(err), "tensorforge_graph_launch");
+  return static_cast<int>(err);
+}
+
+extern "C" int tensorforge_graph_destroy(void* graph_exec) {
+  if (graph_exec == nullptr) return 0;
+  return static_cast<int>(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)));
+}

@@ -211,7 +211,7 @@ class _Stream:
       return False
     if any(t.U.is_sparse() for t in pl.terms):
       return False
-    if (pl.khi - pl.klo) % self.KB != 0:
+    if (pl.khi - pl.klo) % max(2, self.sched.stream_loop_k_block) != 0:
       return False
     if self.sched.stream_loop == 'always':
       return True
@@ -266,8 +266,12 @@ class _Stream:
     # tables: position of every traversal entry in its U array, term in the top bits
     if not self.loop:
       tab = ', '.join(f'{(t_idx << 20) | e[2]}u' for (_, t_idx, e) in self.seq)
+      # guarded: a grouped kernel repeats its members' tables in the same translation unit
+      w(f'#ifndef TF_{self.kname}_UTAB')
+      w(f'#define TF_{self.kname}_UTAB')
       w(f'static __device__ const unsigned {self.kname}_utab[{max(1, self.nu)}] = '
         f'{{{tab or "0u"}}};')
+      w('#endif')
     params = ', '.join(kernel_params(self.prog, self.ctx))
     mw = max(1, self.sched.stream_min_waves)
     w(f'extern "C" __global__ void __launch_bounds__({threads}, {mw}) {self.kname}({params})')

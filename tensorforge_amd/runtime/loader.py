@@ -32,6 +32,13 @@ def runtime():
     lib.tensorforge_copy.restype = ctypes.c_int
     lib.tensorforge_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                      ctypes.c_void_p]
+    for fn in ('tensorforge_graph_begin', 'tensorforge_graph_launch', 'tensorforge_graph_destroy'):
+      getattr(lib, fn).restype = ctypes.c_int
+    lib.tensorforge_graph_begin.argtypes = [ctypes.c_void_p]
+    lib.tensorforge_graph_end.restype = ctypes.c_int
+    lib.tensorforge_graph_end.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+    lib.tensorforge_graph_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.tensorforge_graph_destroy.argtypes = [ctypes.c_void_p]
     _RT = lib
   return _RT
 

@@ -93,6 +93,23 @@ def ader_volume(fp_type=FloatingPointType.DOUBLE, b=56, q=9, sparse_k=True, dens
                   {'b': b, 'q': q, 'sparse_k': sparse_k, 'masks': masks})
 
 
+# uneven batching: orders 2..7 (basis sizes) and their share of the elements
+MIXED_BASES = (4, 10, 20, 35, 56, 84)
+MIXED_WEIGHTS = (0.05, 0.1, 0.2, 0.3, 0.25, 0.1)
+
+
+def mixed_counts(total=500000, seed=3):
+  """Elements per order: ``total`` draws of the order with ``MIXED_WEIGHTS``."""
+  rng = np.random.default_rng(seed)
+  draw = rng.choice(len(MIXED_BASES), size=total, p=MIXED_WEIGHTS)
+  return [int(c) for c in np.bincount(draw, minlength=len(MIXED_BASES))]
+
+
+def ader_mixed(fp_type=FloatingPointType.DOUBLE, bases=MIXED_BASES):
+  """One volume chain per order, dense K_d (b x b), the elastic star pattern."""
+  return [ader_volume(fp_type, b=b, sparse_k=False) for b in bases]
+
+
 # ---------------------------------------------------------------- inputs
 
 def tensor_storage_size(t: Tensor):

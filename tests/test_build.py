@@ -43,6 +43,11 @@ def test_catalog_objects_export_declared_launchers():
     path = catalog.so_path(name)
     assert os.path.exists(path), f'{path} not built'
     exported |= _exports(path)
+  for name in catalog.group_names():
+    for part in ('tuned', 'plain'):
+      path = catalog.group_so_path(name, part)
+      assert os.path.exists(path), f'{path} not built'
+      exported |= _exports(path)
   missing = sorted(declared - exported)
   assert not missing, missing
 
