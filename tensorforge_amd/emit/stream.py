@@ -341,6 +341,8 @@ class _Stream:
     sched = {'kind': 'stream', 'waves_per_block': W, 'k_block': KB, 'elements_per_wave': G,
              'columns': q, 'shared_entries': self.nu, 'lds_bytes': lds_bytes,
              'dpp': self.sched.dpp}
+    if self.loop:
+      sched['loop'] = True
     if self.dma:
       sched['lds_dma_slots'] = self.ns
     fallback = None

@@ -54,9 +54,12 @@ def _atomic_compile(cmd_without_out, out_path, what):
       os.remove(tmp)
 
 
+RUNTIME_SOURCES = ('tensorforge_rt.hip', 'tensorforge_mlp.hip')
+
+
 def build_runtime():
-  src = os.path.join(CSRC_DIR, 'tensorforge_rt.hip')
-  _atomic_compile([hipcc()] + hipcc_flags() + [f'-I{INCLUDE_DIR}', src], runtime_path(),
+  srcs = [os.path.join(CSRC_DIR, s) for s in RUNTIME_SOURCES]
+  _atomic_compile([hipcc()] + hipcc_flags() + [f'-I{INCLUDE_DIR}'] + srcs, runtime_path(),
                   'runtime build')
   return runtime_path()
 

@@ -1,0 +1,61 @@
+"""Feature vector of (chain, batch, candidate) for the cost model: generation-time facts only
+(nothing that needs the kernel compiled or run)."""
+import math
+
+import numpy as np
+
+from ..common.basic_types import FloatingPointType
+
+NAMES = [
+  'log2_batch', 'fp64', 'products', 'max_rows', 'max_cols', 'sum_k', 'log2_shared_nnz',
+  'log2_dense_terms', 'density', 'log2_bytes_per_element', 'log2_flops_per_element',
+  'log2_work', 'intensity', 'kind_linear', 'kind_fused', 'kind_three_index',
+  'em_plain', 'em_column', 'em_stream', 'waves', 'k_block', 'lookahead', 'log2_lds_bytes',
+  'elements_per_block', 'loop_mode', 'log2_fma_sites', 'log2_groups_per_cu',
+]
+
+
+def chain_facts(generator):
+  prog = generator.get_program()
+  rows = cols = sum_k = 0
+  nnz = dense = 0
+  for p in prog.products:
+    rows = max(rows, p.n_ranges[0][1] - p.n_ranges[0][0])
+    if len(p.n_ranges) > 1:
+      cols = max(cols, p.n_ranges[1][1] - p.n_ranges[1][0])
+    for lo, hi in p.k_ranges:
+      sum_k += hi - lo
+    for f in p.factors:
+      if f.src == 'global' and not f.arg.per_element:
+        vol = int(np.prod(f.tensor.shape))
+        nnz += len(f.packed) if f.is_sparse() else vol
+        dense += vol
+  return rows, cols, sum_k, nnz, dense, len(prog.products)
+
+
+def features(generator, candidate, batch, info):
+  em = generator.get_emitted()
+  rows, cols, sum_k, nnz, dense, nprod = chain_facts(generator)
+  bpe = max(1, em.bytes_per_element)
+  flops = max(1, generator.get_program().flops_per_element())
+  sched = em.schedule or {}
+  kind = info.get('kind', 'linear')
+  emitter = 'plain' if candidate.emitter == 'plain' else \
+      ('stream' if sched.get('kind') == 'stream' else 'column')
+  knobs = candidate.knobs()
+  waves = sched.get('waves_per_block', knobs.get('waves_per_block', 1))
+  per_block = max(1, em.elements_per_block)
+  src = generator.get_kernel() or ''
+  sites = src.count('v_fmac_') + src.count('__builtin_fma') + src.count('fma(') + 1
+  groups = batch / per_block / 256.0
+  return [
+    math.log2(batch), float(generator._context.fp_type == FloatingPointType.DOUBLE), nprod,
+    rows, cols, sum_k, math.log2(nnz + 1), math.log2(dense + 1), nnz / max(1, dense),
+    math.log2(bpe), math.log2(flops), math.log2(flops * batch), flops / bpe,
+    float(kind == 'linear'), float(kind == 'fused'), float(kind == 'three_index'),
+    float(emitter == 'plain'), float(emitter == 'column'), float(emitter == 'stream'),
+    waves, sched.get('k_block', knobs.get('k_block', 0)),
+    knobs.get('stream_lds_lookahead', 0) if emitter == 'stream' else 0,
+    math.log2(em.lds_bytes + 1), per_block, float(sched.get('loop', False)),
+    math.log2(sites), math.log2(groups + 1e-3),
+  ]

@@ -28,7 +28,7 @@ def test_runtime_exports_header_symbols():
   path = rt_build.runtime_path()
   assert os.path.exists(path), 'build() first'
   exported = _exports(path)
-  declared = _declared('tensorforge_amd.h')
+  declared = _declared('tensorforge_amd.h') + _declared('tensorforge_mlp.h')
   assert declared, 'no declarations parsed'
   missing = [s for s in declared if s not in exported]
   assert not missing, missing

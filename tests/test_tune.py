@@ -1,0 +1,62 @@
+"""Autotuner pieces that need no GPU: candidate space, random chains, features, evaluation."""
+import numpy as np
+
+from tensorforge_amd import workloads as W
+from tensorforge_amd.common.basic_types import FloatingPointType
+from tensorforge_amd.tune import features as F
+from tensorforge_amd.tune import model as M
+from tensorforge_amd.tune import space
+
+
+def test_candidates_of_the_single_gemm_cover_all_emitters():
+  w = W.gemm()
+  labels = [c.label for c, _ in space.applicable(w.descrs, w.fp_type)]
+  assert 'plain' in labels
+  assert any(l.startswith('stream') for l in labels)
+  assert any(l.startswith('cpl') for l in labels)
+
+
+def test_random_chains_follow_the_stated_distribution():
+  chains = space.random_chains(60, seed=11)
+  again = space.random_chains(60, seed=11)
+  assert [c[3] for c in chains] == [c[3] for c in again]
+  kinds = {c[3]['kind'] for c in chains}
+  assert kinds == {'linear', 'fused', 'three_index'}
+  for descrs, fp, batch, info in chains:
+    assert 1000 <= batch <= 10 ** 6
+    assert all(4 <= d <= 128 for d in info['dims'][:2])
+    assert fp in (FloatingPointType.FLOAT, FloatingPointType.DOUBLE)
+    if info['kind'] == 'linear':
+      assert 2 <= info['gemms'] <= 5
+
+
+def test_feature_vectors_have_the_named_length():
+  w = W.ader_volume()
+  for c, g in space.applicable(w.descrs, w.fp_type)[:4]:
+    v = F.features(g, c, 262144, {'kind': 'fused'})
+    assert len(v) == len(F.NAMES) and all(np.isfinite(v))
+
+
+class _Oracle:
+  """A 'model' that predicts the measured times exactly: picks must be the sweep's best."""
+
+  def __init__(self, recs):
+    self.t = {tuple(r['features']): np.log(r['time']) for r in recs}
+
+  def predict(self, X):
+    return np.array([self.t[tuple(x)] for x in X.tolist()])
+
+
+def test_evaluation_measures_slowdown_against_the_sweep():
+  recs = []
+  for chain in range(4):
+    for j, label in enumerate(['plain', 'cpl_w1_k16', 'stream_w8_k8_a2']):
+      recs.append({'chain': chain, 'label': label, 'time': 1.0 + j + chain,
+                   'features': [float(chain), float(j)]})
+  s = M.evaluate(_Oracle(recs), recs)
+  assert s['model_pick']['mean_slowdown'] == 0.0
+  assert s['pruned_search']['within_5pct'] == 1.0
+  # the default choice is the streaming schedule here: the slowest
+  assert s['default']['mean_slowdown'] > 0.5
+  train, test = M.split(recs, 0.25, seed=1)
+  assert {r['chain'] for r in train}.isdisjoint({r['chain'] for r in test})
