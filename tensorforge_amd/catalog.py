@@ -180,6 +180,11 @@ for _emitter in ('plain', 'tuned'):
     register(f'ader_dense_{_fpn}_{_emitter}')(
       lambda fp=_fp, e=_emitter: _wl_entry(W.ader_volume(fp, sparse_k=False), e))
 
+# precision study: order 7 (84 basis functions), dense K_d, FP64 inputs and outputs
+for _emitter in ('plain', 'tuned', 'mfma_fp32', 'mfma_bf16x3'):
+  register(f'ader7_f64_{_emitter}')(
+    lambda e=_emitter: _wl_entry(W.ader_volume(F64, b=84, sparse_k=False), e))
+
 
 # --------------------------------------------------------------- test chains
 # Shapes follow the reference's examples (example/*.py) and test suites (tests/gemms/testsuites).

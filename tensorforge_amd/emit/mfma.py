@@ -1,0 +1,391 @@
+"""Matrix-core forms of the streaming chains (precision study): FP32 MFMA with FP32 accumulation,
+and split BF16x3 MFMA with a compensating second accumulator.
+
+Applies to the chains the streaming schedule recognises, with dense shared operands::
+
+    D_e = sum_t U_t · (L_e · W_t,e)          (U_t shared b x b, L_e per element b x q,
+                                               W_t,e per element q x q, dense or sparse)
+
+The per-element temporaries ``L_e · W_t,e`` (the star-matrix step) stay in FP64 on the vector
+units; the products with the shared matrices run on MFMA. Inputs and outputs keep the FP64 layout
+and launcher of the chain, so the launcher is a drop-in for the FP64 one.
+
+Mapping: a workgroup of 4 wavefronts stays resident; each wavefront owns ``64 // q`` elements, i.e.
+``G·q`` output columns = ``ceil(G·q / 16)`` MFMA column tiles, and all row tiles of them. The
+shared matrices are converted once per workgroup into LDS in MFMA A-operand order. Per block of 16
+contraction rows and per term, each lane forms its column of the temporary (FP64, operands of other
+columns through ``ds_bpermute``), rounds or splits it into a per-wavefront LDS tile in B-operand
+order, and the wavefront issues the MFMAs of that block.
+
+Split BF16x3 (the three-product scheme): ``x = hi + lo`` with ``hi = bf16(x)``,
+``lo = bf16(x - hi)``; ``a·b ~ hi_a·hi_b + (hi_a·lo_b + lo_a·hi_b)``; the first product
+accumulates in one FP32 accumulator, the two correction products in a second, and the two are
+added in FP64 at the end, so the small corrections are not absorbed by the rounding of the large
+sum.
+"""
+from ..common.basic_types import Addressing, FloatingPointType
+from . import ir
+from .launcher import Emitted, Writer, kernel_params
+from .traffic import element_traffic
+
+WAVE = 64
+WAVES = 4
+KB = 16            # contraction rows per block
+
+
+def unsupported_reason(program, pl):
+  if pl is None:
+    return 'not a streaming chain'
+  if program.fp_type != FloatingPointType.DOUBLE:
+    return 'the matrix-core forms take FP64 chains'
+  if any(t.kind != 'fused' for t in pl.terms):
+    return 'a term without a per-element temporary'
+  if any(t.U.is_sparse() for t in pl.terms):
+    return 'sparse shared operand'
+  if pl.L is None or pl.L.arg.addressing != Addressing.STRIDED:
+    return 'streamed operand is not strided'
+  if pl.l_cols[1] - pl.l_cols[0] != pl.q or pl.q > 16:
+    return 'temporary columns'
+  if pl.rows[1] - pl.rows[0] > 128 or pl.khi - pl.klo > 128:
+    return 'more than 128 rows'
+  if pl.init == 'load' and pl.beta != 1.0:
+    return 'scaled accumulation'
+  return None
+
+
+class _Mfma:
+  def __init__(self, program, context, base_name, pl, mode):
+    self.prog, self.ctx, self.pl, self.mode = program, context, pl, mode
+    self.kname = f'kernel_{base_name}'
+    self.w = Writer()
+    self.q = pl.q
+    self.G = WAVE // pl.q
+    self.cols = self.G * pl.q
+    self.CT = (self.cols + 15) // 16
+    self.rows = pl.rows[1] - pl.rows[0]
+    self.RT = (self.rows + 15) // 16
+    self.K = pl.khi - pl.klo
+    self.NKB = (self.K + KB - 1) // KB
+    self.kstep = 4 if mode == 'fp32' else 16
+    self.KS = self.NKB * (KB // self.kstep)
+    self.T = len(pl.terms)
+
+  # --------------------------------------------------------------- per-lane star slots
+  def slot_tables(self, t):
+    """[(m per column, position per column or -1)] for term t's weights."""
+    prod = t.producer
+    mlo, mhi = prod.k_ranges[0]
+    c_lo = prod.n_ranges[1][0]
+    rk, rn = t.w_dims
+    cols = []
+    for c in range(self.q):
+      ms = []
+      for m in range(mlo, mhi):
+        idx = [0, 0]
+        idx[rk], idx[rn] = m, c_lo + c
+        pos = t.W.packed.get(tuple(idx)) if t.W.is_sparse() else t.W.offset(idx)
+        if pos is not None:
+          ms.append((m, pos))
+      cols.append(ms)
+    n = max(len(c) for c in cols)
+    out = []
+    for s in range(n):
+      m = [c[s][0] if s < len(c) else (c[0][0] if c else mlo) for c in cols]
+      p = [c[s][1] if s < len(c) else -1 for c in cols]
+      out.append((m, p))
+    return out
+
+  @staticmethod
+  def select(values):
+    if all(v == values[0] for v in values):
+      return str(values[0])
+    expr = str(values[-1])
+    for c in range(len(values) - 2, -1, -1):
+      expr = f'(col == {c} ? {values[c]} : {expr})'
+    return expr
+
+  # --------------------------------------------------------------- emission
+  def emit(self):
+    w = self.w
+    pl = self.pl
+    T, KS, RT, CT = self.T, self.KS, self.RT, self.CT
+    bf16 = self.mode == 'bf16x3'
+    threads = WAVE * WAVES
+    frag = T * KS * RT * WAVE
+    if bf16:
+      kbytes = frag * 8 * 2
+      pitch = 20                          # bf16 per column of the B tile (conflict-free reads)
+      tbytes = 64 * pitch * 2 * 2
+    else:
+      kbytes = frag * 4
+      pitch = 17
+      tbytes = 64 * pitch * 4
+    lds_bytes = kbytes + WAVES * tbytes
+    self.lds_bytes = lds_bytes
+    w('typedef float tf_f4 __attribute__((ext_vector_type(4)));')
+    w('typedef short tf_s4 __attribute__((ext_vector_type(4)));')
+    w('#ifndef TF_BF16_HELPERS')
+    w('#define TF_BF16_HELPERS')
+    w('__device__ __forceinline__ unsigned tf_bf16(float x) {')
+    w('  unsigned u = __float_as_uint(x);')
+    w('  u += 0x7fffu + ((u >> 16) & 1u);')
+    w('  return u >> 16;')
+    w('}')
+    w('__device__ __forceinline__ float tf_bf16_f(unsigned h) { return __uint_as_float(h << 16); }')
+    w('#endif')
+    params = ', '.join(kernel_params(self.prog, self.ctx))
+    w(f'extern "C" __global__ void __launch_bounds__({threads}, 1) {self.kname}({params})')
+    w('{')
+    w._depth += 1
+    w(f'// matrix-core form ({self.mode}): {WAVES} wavefronts, {self.G} elements x {self.q} '
+      f'columns each, {RT}x{CT} tiles of 16x16, contraction blocks of {KB}')
+    w('extern __shared__ __align__(16) unsigned char lds_raw[];')
+    if bf16:
+      w('uint2* __restrict__ kh = reinterpret_cast<uint2*>(lds_raw);')
+      w(f'uint2* __restrict__ kl = kh + {frag};')
+    else:
+      w('float* __restrict__ kf = reinterpret_cast<float*>(lds_raw);')
+    w('const int wave = threadIdx.x / 64;')
+    w('const int lane = threadIdx.x % 64;')
+    if bf16:
+      w(f'unsigned short* __restrict__ th = reinterpret_cast<unsigned short*>(lds_raw + {kbytes} + '
+        f'wave * {tbytes});')
+      w(f'unsigned short* __restrict__ tl = th + {64 * pitch};')
+    else:
+      w(f'float* __restrict__ tf = reinterpret_cast<float*>(lds_raw + {kbytes} + wave * {tbytes});')
+    self.stage_shared(frag, threads)
+    w('__syncthreads();')
+    q, G = self.q, self.G
+    w(f'const int slot = lane / {q} < {G} ? lane / {q} : {G - 1};')
+    w(f'const int col = lane / {q} < {G} ? lane - slot * {q} : 0;')
+    w(f'const int gcol = slot * {q} + col;')
+    # bpermute source lanes of every slot table
+    self.tables = []
+    src_names = {}
+    for ti, t in enumerate(pl.terms):
+      tabs = []
+      for si, (m, p) in enumerate(self.slot_tables(t)):
+        key = tuple(m)
+        if key not in src_names:
+          v = f'src{len(src_names)}'
+          w(f'const int {v} = (slot * {q} + ({self.select(list(m))} - {pl.l_cols[0]})) * 4;')
+          src_names[key] = v
+        pv = f'wp{ti}_{si}'
+        w(f'const int {pv} = {self.select(p)};')
+        tabs.append((src_names[key], pv))
+      self.tables.append(tabs)
+    w(f'const size_t ngroups = (numElements + {G - 1}) / {G};')
+    w(f'const size_t stride = (size_t)gridDim.x * {WAVES};')
+    with w.block('for (size_t grp = (size_t)blockIdx.x * ' + str(WAVES) + ' + wave; grp < ngroups; '
+                 'grp += stride)'):
+      self.group(pitch)
+    w._depth -= 1
+    w('}')
+    traffic = element_traffic(self.prog)
+    sched = {'kind': f'mfma_{self.mode}', 'waves_per_block': WAVES, 'elements_per_wave': G,
+             'row_tiles': RT, 'column_tiles': CT, 'k_block': KB, 'lds_bytes': lds_bytes}
+    return Emitted(kernel_name=self.kname, kernel_src=w.src(), block=(threads, 1, 1),
+                   elements_per_block=G * WAVES, lds_bytes=lds_bytes, emitter='mfma',
+                   schedule=sched, bytes_per_element=traffic.per_element,
+                   shared_bytes=traffic.shared, grid_override='tf_grid',
+                   extra_launch_setup=self.grid_setup(lds_bytes, threads))
+
+  def shared_value(self, t_expr, row, k):
+    """FP64 value of U_t(row, k) (relative indices), 0 outside the ranges."""
+    pl = self.pl
+    expr = '0.0'
+    for ti in range(self.T - 1, -1, -1):
+      t = pl.terms[ti]
+      U = t.U
+      nd, kd = t.u_dims
+      pos = (f'({row} + {pl.rows[0] - U.origin[nd]}) * {U.strides[nd]} + ({k} + '
+             f'{pl.klo - U.origin[kd]}) * {U.strides[kd]}')
+      expr = f'({t_expr} == {ti} ? (double){U.arg.name}[{pos}] : {expr})'
+    return f'(({row}) < {self.rows} && ({k}) < {self.K} ? {expr} : 0.0)'
+
+  def stage_shared(self, frag, threads):
+    w = self.w
+    RT, KS = self.RT, self.KS
+    with w.block(f'for (int e = threadIdx.x; e < {frag}; e += {threads})'):
+      w('const int l = e % 64;')
+      w(f'const int rt = (e / 64) % {RT};')
+      w(f'const int ks = (e / {64 * RT}) % {KS};')
+      w(f'const int t = e / {64 * RT * KS};')
+      w('const int row = rt * 16 + l % 16;')
+      if self.mode == 'fp32':
+        w(f'kf[e] = (float){self.shared_value("t", "row", "ks * 4 + l / 16")};')
+      else:
+        w('unsigned hi[4], lo[4];')
+        with w.block('for (int j = 0; j < 4; ++j)'):
+          w(f'const float x = (float){self.shared_value("t", "row", "ks * 16 + 4 * (l / 16) + j")};')
+          w('hi[j] = tf_bf16(x);')
+          w('lo[j] = tf_bf16(x - tf_bf16_f(hi[j]));')
+        w('kh[e] = make_uint2(hi[0] | (hi[1] << 16), hi[2] | (hi[3] << 16));')
+        w('kl[e] = make_uint2(lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16));')
+
+  def group(self, pitch):
+    w = self.w
+    pl = self.pl
+    q, G, RT, CT = self.q, self.G, self.RT, self.CT
+    bf16 = self.mode == 'bf16x3'
+    L = pl.L
+    w(f'const size_t e0 = grp * {G};')
+    w('const size_t batchId = e0 + slot;')
+    w(f'const size_t eSafe = batchId < numElements ? batchId : 0;')
+    # per-element operands of this lane's element
+    for a in self.prog.args:
+      if a.kind != ir.SRC_GLOBAL or not a.per_element:
+        continue
+      ty = f'{"const " if a.const else ""}{a.dtype}'
+      w(f'{ty}* __restrict__ g_{a.name} = {a.name} + eSafe * {a.element_stride}ull + '
+        f'{a.name}_extraOffset;')
+    w(f'const double* __restrict__ lq = g_{L.arg.name} + (col + {pl.l_cols[0] - L.origin[1]}) * '
+      f'{L.strides[1]};')
+    # weights of every slot of every term (FP64)
+    for ti, t in enumerate(pl.terms):
+      for si, (src, pv) in enumerate(self.tables[ti]):
+        w(f'const double w{ti}_{si} = {pv} >= 0 ? (double)g_{t.W.arg.name}[{pv} < 0 ? 0 : {pv}] '
+          f': 0.0;')
+    w('tf_f4 acc[' + str(RT * CT) + '];')
+    if bf16:
+      w('tf_f4 acr[' + str(RT * CT) + '];')
+    w('#pragma unroll')
+    with w.block(f'for (int i = 0; i < {RT * CT}; ++i)'):
+      w('acc[i] = tf_f4{0.f, 0.f, 0.f, 0.f};')
+      if bf16:
+        w('acr[i] = tf_f4{0.f, 0.f, 0.f, 0.f};')
+    w('#pragma unroll 1')
+    with w.block(f'for (int kb = 0; kb < {self.NKB}; ++kb)'):
+      w(f'const int k0 = kb * {KB};')
+      self.qexpr = (f'(k0 + j < {self.K} ? lq[(k0 + j + {pl.klo - L.origin[0]}) * {L.strides[0]}] '
+                    f': 0.0)')
+      for ti in range(self.T):
+        self.term_block(ti, pitch)
+    self.store()
+
+  def term_block(self, ti, pitch):
+    w = self.w
+    RT, CT = self.RT, self.CT
+    bf16 = self.mode == 'bf16x3'
+    tabs = self.tables[ti]
+    w(f'// term {ti}: temporary rows (FP64) into the B tile')
+    w('#pragma unroll 4')
+    with w.block(f'for (int j = 0; j < {KB}; ++j)'):
+      w(f'const double qv = {self.qexpr};')
+      w('const int qlo = __double2loint(qv);')
+      w('const int qhi = __double2hiint(qv);')
+      w('double tv = 0.0;')
+      for si, (src, pv) in enumerate(tabs):
+        w(f'tv = __builtin_fma(__hiloint2double(__builtin_amdgcn_ds_bpermute({src}, qhi), '
+          f'__builtin_amdgcn_ds_bpermute({src}, qlo)), w{ti}_{si}, tv);')
+      if bf16:
+        w('const float tx = (float)tv;')
+        w('const unsigned h = tf_bf16(tx);')
+        w(f'th[gcol * {pitch} + j] = (unsigned short)h;')
+        w(f'tl[gcol * {pitch} + j] = (unsigned short)tf_bf16(tx - tf_bf16_f(h));')
+      else:
+        w(f'tf[gcol * {pitch} + j] = (float)tv;')
+    w('__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");')
+    w('__builtin_amdgcn_wave_barrier();')
+    w('__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");')
+    nks = KB // self.kstep
+    frag = f'((({ti} * {self.KS} + kb * {nks} + ks) * {RT} + rt) * 64 + lane)'
+    w('#pragma unroll')
+    with w.block(f'for (int ks = 0; ks < {nks}; ++ks)'):
+      if bf16:
+        w(f'tf_s4 bh[{CT}], bl[{CT}];')
+        w('#pragma unroll')
+        with w.block(f'for (int ct = 0; ct < {CT}; ++ct)'):
+          w(f'const int o = (ct * 16 + lane % 16) * {pitch} + 4 * (lane / 16);')
+          w('const uint2 vh = *reinterpret_cast<const uint2*>(th + o);')
+          w('const uint2 vl = *reinterpret_cast<const uint2*>(tl + o);')
+          w('bh[ct] = __builtin_bit_cast(tf_s4, vh);')
+          w('bl[ct] = __builtin_bit_cast(tf_s4, vl);')
+        w('#pragma unroll')
+        with w.block(f'for (int rt = 0; rt < {RT}; ++rt)'):
+          w(f'const tf_s4 ah = __builtin_bit_cast(tf_s4, kh[{frag}]);')
+          w(f'const tf_s4 al = __builtin_bit_cast(tf_s4, kl[{frag}]);')
+          w('#pragma unroll')
+          with w.block(f'for (int ct = 0; ct < {CT}; ++ct)'):
+            w(f'acc[rt * {CT} + ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[ct], '
+              f'acc[rt * {CT} + ct], 0, 0, 0);')
+            w(f'acr[rt * {CT} + ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[ct], '
+              f'acr[rt * {CT} + ct], 0, 0, 0);')
+            w(f'acr[rt * {CT} + ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[ct], '
+              f'acr[rt * {CT} + ct], 0, 0, 0);')
+      else:
+        w(f'float bv[{CT}];')
+        w('#pragma unroll')
+        with w.block(f'for (int ct = 0; ct < {CT}; ++ct)'):
+          w(f'bv[ct] = tf[(ct * 16 + lane % 16) * {pitch} + ks * 4 + lane / 16];')
+        w('#pragma unroll')
+        with w.block(f'for (int rt = 0; rt < {RT}; ++rt)'):
+          w(f'const float av = kf[{frag}];')
+          w('#pragma unroll')
+          with w.block(f'for (int ct = 0; ct < {CT}; ++ct)'):
+            w(f'acc[rt * {CT} + ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[ct], '
+              f'acc[rt * {CT} + ct], 0, 0, 0);')
+    # the next term rewrites the tile: its reads above are complete (consumed by the MFMAs)
+    w('__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");')
+    w('__builtin_amdgcn_wave_barrier();')
+    w('__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");')
+
+  def store(self):
+    w = self.w
+    pl = self.pl
+    q, G, RT, CT = self.q, self.G, self.RT, self.CT
+    dp = pl.dest_product
+    o0, o1 = dp.dest_origin
+    s0, s1 = dp.dest_strides
+    d = pl.dest
+    w('#pragma unroll')
+    with w.block(f'for (int rt = 0; rt < {RT}; ++rt)'):
+      w('#pragma unroll')
+      with w.block(f'for (int ct = 0; ct < {CT}; ++ct)'):
+        w('const int g = ct * 16 + lane % 16;')
+        w(f'const int so = g / {q};')
+        w(f'const int co = g % {q};')
+        w('const size_t eo = e0 + so;')
+        w(f'if (g >= {G * q} || eo >= numElements) continue;')
+        w('if (flags != nullptr && flags[eo] == 0u) continue;')
+        w(f'{d.dtype}* __restrict__ dst = {d.name} + eo * {d.element_stride}ull + '
+          f'{d.name}_extraOffset;')
+        w('#pragma unroll')
+        with w.block('for (int jj = 0; jj < 4; ++jj)'):
+          w('const int row = rt * 16 + 4 * (lane / 16) + jj;')
+          w(f'if (row >= {self.rows}) continue;')
+          w(f'const size_t off = (size_t)(row + {pl.rows[0] - o0}) * {s0} + '
+            f'(size_t)(co + {pl.cols[0] - o1}) * {s1};')
+          if self.mode == 'bf16x3':
+            w(f'double v = (double)acc[rt * {CT} + ct][jj] + (double)acr[rt * {CT} + ct][jj];')
+          else:
+            w(f'double v = (double)acc[rt * {CT} + ct][jj];')
+          if pl.init == 'load':
+            w('v += dst[off];')
+          w('dst[off] = v;')
+
+  def grid_setup(self, lds_bytes, threads):
+    G = self.G
+    return (f'static int tf_resident = 0;\n'
+            f'if (tf_resident == 0) {{\n'
+            f'  int dev = 0, cus = 0, per_cu = 0;\n'
+            f'  (void)hipGetDevice(&dev);\n'
+            f'  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);\n'
+            f'  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&{self.kname}), '
+            f'hipFuncAttributeMaxDynamicSharedMemorySize, {lds_bytes});\n'
+            f'  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, {self.kname}, {threads}, '
+            f'{lds_bytes});\n'
+            f'  tf_resident = (cus > 0 ? cus : 256) * (per_cu > 0 ? per_cu : 1);\n'
+            f'}}\n'
+            f'const size_t tf_groups = (numElements + {G * WAVES - 1}) / {G * WAVES};\n'
+            f'const unsigned tf_grid = static_cast<unsigned>(tf_groups < (size_t)tf_resident ? '
+            f'tf_groups : (size_t)tf_resident);')
+
+
+def emit(program, context, base_name, mode):
+  from . import stream
+  pl, why = stream.plan(program)
+  reason = unsupported_reason(program, pl) if pl is not None else why
+  if reason is not None:
+    return None, reason
+  return _Mfma(program, context, base_name, pl, mode).emit(), None

@@ -149,6 +149,12 @@ class Generator:
     from ..emit import plain, tuned, launcher
     choice = self._emitter_choice
     emitted = None
+    if choice in ('mfma_fp32', 'mfma_bf16x3'):
+      from ..emit import mfma
+      emitted, reason = mfma.emit(self._program, self._context, self._base_kernel_name,
+                                  choice[len('mfma_'):])
+      if emitted is None:
+        raise GenerationError(f'{choice} cannot generate this chain: {reason}')
     if choice in ('auto', 'tuned'):
       reason = tuned.unsupported_reason(self._program, self._context, self._schedule)
       if reason is None:
@@ -157,7 +163,7 @@ class Generator:
       elif choice == 'tuned':
         raise GenerationError(f'tuned emitter cannot generate this chain: {reason}')
     if emitted is None:
-      if choice not in ('auto', 'tuned', 'plain'):
+      if choice not in ('auto', 'tuned', 'plain', 'mfma_fp32', 'mfma_bf16x3'):
         raise ValueError(f'unknown emitter {choice!r}')
       emitted = plain.emit(self._program, self._context, self._base_kernel_name,
                            self._num_threads, self._thread_block_policy_type)

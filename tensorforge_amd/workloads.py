@@ -131,6 +131,11 @@ def gemm_inputs(w: Workload, num_elements, seed=0, dtype=np.float64):
   return {'A': A.astype(dtype), 'B': B.astype(dtype), 'C': C.astype(dtype)}
 
 
+def log_uniform_q(rng, size, lo=-6.0, hi=3.0):
+  """Degrees of freedom with magnitudes log-uniform in [10^lo, 10^hi] and random sign."""
+  return np.where(rng.random(size) < 0.5, -1.0, 1.0) * 10.0 ** rng.uniform(lo, hi, size)
+
+
 def ader_inputs(w: Workload, num_elements, seed=2, dtype=np.float64, q_scale=None):
   """Q ~ N(0,1) (or log-uniform magnitudes when ``q_scale=(lo_exp, hi_exp)``), star values
   U[-2,2] packed at the start of each 81-entry slot, K values U[-1,1] packed in CSR order."""
