@@ -33,7 +33,8 @@ def parse():
   ap.add_argument('--elements', type=int, default=262144)
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--cpu-seconds', type=float, default=15.0)
-  ap.add_argument('--only', default='', help='comma list of legs (tuned,plain,f32,copy,mixed)')
+  ap.add_argument('--only', default='',
+                  help='comma list of legs (tuned,plain,f32,copy,mixed,precision,tuner)')
   return ap.parse_args()
 
 
@@ -201,6 +202,62 @@ def mixed_leg(device, steps, warmup, graph_steps=10):
   return res
 
 
+def _normwise(got, ref, vol):
+  import numpy as np
+  g = got.reshape(-1, vol)
+  r = ref.reshape(-1, vol)
+  den = np.maximum(np.abs(r).max(axis=1), np.finfo(np.float64).tiny)
+  return float((np.abs(g - r).max(axis=1) / den).max())
+
+
+def precision_leg(device, steps, warmup, n=262144):
+  """Order-7 chain (84 basis functions, dense K_d), |Q| log-uniform in [1e-6, 1e3]: FP64
+  vector (streaming schedule), FP32 MFMA and split BF16x3 MFMA. Error: normwise per element
+  against the FP64 kernel's result on all elements (the FP64 kernel itself is checked against
+  the FP64 oracle to 1e-12 by the GPU tests)."""
+  import numpy as np
+  import torch
+  from tensorforge_amd import catalog
+  from tensorforge_amd import workloads as W
+  b, q = 84, 9
+  rng = np.random.default_rng(4)
+  host = {'Q': W.log_uniform_q(rng, n * b * q)}
+  for d in range(3):
+    host[f'K{d + 1}'] = rng.uniform(-1, 1, b * b)
+    sv = np.zeros((n, q * q))
+    sv[:, :24] = rng.uniform(-2, 2, (n, 24))
+    host[f'S{d + 1}'] = sv.reshape(-1)
+  bufs = {k: torch.from_numpy(v).to(device) for k, v in host.items()}
+  vol = b * q
+  res = {'workload': 'order 7 (84x9), dense K_d, 262144 elements, |Q| log-uniform [1e-6, 1e3]',
+         'error': 'normwise per element vs the FP64 kernel, all elements'}
+  out64 = None
+  for name, label in (('ader7_f64_tuned', 'fp64_vector'), ('ader7_f64_mfma_fp32', 'fp32_mfma'),
+                      ('ader7_f64_mfma_bf16x3', 'bf16x3_mfma')):
+    _, k = catalog.load(name)
+    bufs['Qp'] = torch.zeros(n * vol, dtype=torch.float64, device=device)
+    t = time_kernel(k, bufs, n, steps, warmup)
+    got = bufs['Qp'].cpu().numpy()
+    if out64 is None:
+      out64 = got
+    res[label] = {'value': n / t, 'unit': 'elements/s', 'ms_per_step': t * 1e3,
+                  'err_vs_fp64_kernel': _normwise(got, out64, vol)}
+  del bufs
+  return res
+
+
+def tuner_leg():
+  """Cost model vs the exhaustive sweep: five-fold cross-validation by chain over the
+  committed timing database (70 random chains, every applicable candidate timed on MI355X);
+  the network trains on the GPU in the project's HIP kernels."""
+  from tensorforge_amd.tune import model as M
+  path = os.path.join(ROOT, 'tensorforge_amd', 'tune', 'data', 'timings_mi355x.jsonl')
+  recs = M.load(path)
+  s = M.cross_validate(recs, folds=5, seed=0, prune_k=3, epochs=800, centered=True)
+  s['candidates_per_chain'] = len(recs) / s['chains']
+  return s
+
+
 def profiled_traffic(kernel_name, schedule):
   """HBM bytes per launch of ``kernel_name`` from the committed rocprofv3 counter summary
   (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor the same counter reads for
@@ -293,7 +350,8 @@ def main():
   import torch
   from tensorforge_amd import catalog
   device = torch.device('cuda:0')
-  legs = set(args.only.split(',')) if args.only else {'tuned', 'plain', 'f32', 'copy', 'mixed'}
+  legs = set(args.only.split(',')) if args.only else {'tuned', 'plain', 'f32', 'copy', 'mixed',
+                                                      'precision', 'tuner'}
   n = args.elements
   out = {}
 
@@ -351,6 +409,12 @@ def main():
 
   if 'mixed' in legs:
     out['mixed_orders'] = mixed_leg(device, args.steps, args.warmup)
+
+  if 'precision' in legs:
+    out['precision_order7'] = precision_leg(device, max(3, args.steps // 5), 2)
+
+  if 'tuner' in legs:
+    out['cost_model'] = tuner_leg()
 
   measured_bw = copy_bandwidth(device) if 'copy' in legs else None
 
