@@ -1145,5 +1145,51 @@ class _Stream:
       w(f'{acc} = __builtin_fma({b}, {r}, {acc});')
 
 
+def _row_passes(pl, schedule):
+  """Row ranges of the passes: one pass unless the destination has more rows than
+  ``stream_pass_rows`` and its shared operands are dense (the runtime block loop)."""
+  rows = pl.rows[1] - pl.rows[0]
+  limit = schedule.stream_pass_rows
+  if limit <= 0 or rows <= limit or any(t.U.is_sparse() for t in pl.terms):
+    return [pl.rows]
+  n = (rows + limit - 1) // limit
+  step = (rows + n - 1) // n
+  return [(r, min(r + step, pl.rows[1])) for r in range(pl.rows[0], pl.rows[1], step)]
+
+
 def emit(program, context, base_name, schedule, pl) -> Emitted:
-  return _Stream(program, context, base_name, schedule, pl).emit()
+  passes = _row_passes(pl, schedule)
+  if len(passes) == 1:
+    return _Stream(program, context, base_name, schedule, pl).emit()
+  # several passes over the contraction, each for a band of destination rows: the accumulator
+  # of a band fits the registers and the band's shared rows fit LDS; the streamed operand is
+  # read once per pass
+  import dataclasses
+  parts = []
+  for r0, r1 in passes:
+    sub = dataclasses.replace(pl, rows=(r0, r1),
+                              terms=[dataclasses.replace(t, entries=[]) for t in pl.terms])
+    parts.append(_Stream(program, context, base_name, schedule, sub).emit())
+  first = parts[0]
+  kname = first.kernel_name
+  lines = first.kernel_src.rstrip('\n').split('\n')
+  sig = next(i for i, l in enumerate(lines) if l.startswith('extern "C" __global__'))
+  head = lines[:sig + 2]
+  body = []
+  for k, part in enumerate(parts):
+    pl_lines = part.kernel_src.rstrip('\n').split('\n')
+    psig = next(i for i, l in enumerate(pl_lines) if l.startswith('extern "C" __global__'))
+    body.append(f'  // rows [{passes[k][0]}, {passes[k][1]})')
+    body.append('  {')
+    body += ['  ' + l for l in pl_lines[psig + 2:-1]]
+    body.append('  }')
+    if k + 1 < len(parts):
+      body.append('  __syncthreads();')
+  src = '\n'.join(head + body + ['}']) + '\n'
+  lds = max(p.lds_bytes for p in parts)
+  sched = dict(first.schedule)
+  sched['row_passes'] = len(parts)
+  sched['lds_bytes'] = lds
+  return dataclasses.replace(first, kernel_src=src, lds_bytes=lds, schedule=sched,
+                             extra_launch_setup=first.extra_launch_setup.replace(
+                                 str(first.lds_bytes), str(lds)))

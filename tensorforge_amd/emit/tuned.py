@@ -75,6 +75,8 @@ class Schedule:
   stream_k_block: int = 8
   # contraction block of the runtime block loop (dense shared operands with many entries)
   stream_loop_k_block: int = 4
+  # destination rows per pass of the runtime block loop (0: one pass)
+  stream_pass_rows: int = 64
   stream_min_waves: int = 2
   # runtime loop over contraction blocks: 'auto' (dense shared operands with many entries),
   # 'always', 'never'; where the shared entries are read from: 'lds', 'global', 'auto'
