@@ -120,7 +120,7 @@ class _Mfma:
       kbytes = frag * 4
       pitch = 17
       tbytes = 64 * pitch * 4
-    lds_bytes = kbytes + WAVES * tbytes
+    lds_bytes = kbytes + WAVES * 2 * tbytes
     self.lds_bytes = lds_bytes
     w('typedef float tf_f4 __attribute__((ext_vector_type(4)));')
     w('typedef short tf_s4 __attribute__((ext_vector_type(4)));')
@@ -147,12 +147,13 @@ class _Mfma:
       w('float* __restrict__ kf = reinterpret_cast<float*>(lds_raw);')
     w('const int wave = threadIdx.x / 64;')
     w('const int lane = threadIdx.x % 64;')
-    if bf16:
-      w(f'unsigned short* __restrict__ th = reinterpret_cast<unsigned short*>(lds_raw + {kbytes} + '
-        f'wave * {tbytes});')
-      w(f'unsigned short* __restrict__ tl = th + {64 * pitch};')
-    else:
-      w(f'float* __restrict__ tf = reinterpret_cast<float*>(lds_raw + {kbytes} + wave * {tbytes});')
+    for b in range(2):
+      base = f'lds_raw + {kbytes} + (wave * 2 + {b}) * {tbytes}'
+      if bf16:
+        w(f'unsigned short* __restrict__ th{b} = reinterpret_cast<unsigned short*>({base});')
+        w(f'unsigned short* __restrict__ tl{b} = th{b} + {64 * pitch};')
+      else:
+        w(f'float* __restrict__ tf{b} = reinterpret_cast<float*>({base});')
     self.stage_shared(frag, threads)
     w('__syncthreads();')
     q, G = self.q, self.G
@@ -254,81 +255,110 @@ class _Mfma:
       w('acc[i] = tf_f4{0.f, 0.f, 0.f, 0.f};')
       if bf16:
         w('acr[i] = tf_f4{0.f, 0.f, 0.f, 0.f};')
-    w('#pragma unroll 1')
-    with w.block(f'for (int kb = 0; kb < {self.NKB}; ++kb)'):
-      w(f'const int k0 = kb * {KB};')
-      self.qexpr = (f'(k0 + j < {self.K} ? lq[(k0 + j + {pl.klo - L.origin[0]}) * {L.strides[0]}] '
-                    f': 0.0)')
-      for ti in range(self.T):
-        self.term_block(ti, pitch)
+    # software pipeline over the (block, term) steps: the MFMAs of step s read B tile s % 2
+    # while the lanes form step s + 1's temporary rows into the other tile; the streamed rows of
+    # block kb + 1 are loaded when block kb starts
+    steps = [(kb, t) for kb in range(self.NKB) for t in range(self.T)]
+    w(f'double qa[{KB}], qb[{KB}];')
+    self.load_rows(0, 'qa')
+    self.t_rows(steps[0], 'qa', 0, pitch, range(KB))
+    self.tile_fence()
+    for si, (kb, t) in enumerate(steps):
+      qcur = 'qa' if kb % 2 == 0 else 'qb'
+      qnxt = 'qb' if kb % 2 == 0 else 'qa'
+      w(f'// step {si}: block {kb}, term {t}')
+      if t == 0 and kb + 1 < self.NKB:
+        self.load_rows(kb + 1, qnxt)
+      nxt = steps[si + 1] if si + 1 < len(steps) else None
+      mf = self.mfma_list(kb, t, si % 2, pitch)
+      per = (len(mf) + KB - 1) // KB
+      for j in range(KB):
+        if nxt is not None:
+          self.t_rows(nxt, qcur if nxt[0] == kb else qnxt, (si + 1) % 2, pitch, [j])
+        for line in mf[j * per:(j + 1) * per]:
+          w(line)
+        # keeps the compiler from hoisting every fragment load and temporary row of the step
+        w('__builtin_amdgcn_sched_barrier(0);')
+      self.tile_fence()
     self.store()
 
-  def term_block(self, ti, pitch):
+  def load_rows(self, kb, dst):
     w = self.w
-    RT, CT = self.RT, self.CT
-    bf16 = self.mode == 'bf16x3'
+    pl = self.pl
+    L = pl.L
+    for j in range(KB):
+      k = kb * KB + j
+      if k < self.K:
+        w(f'{dst}[{j}] = lq[{(k + pl.klo - L.origin[0]) * L.strides[0]}];')
+      else:
+        w(f'{dst}[{j}] = 0.0;')
+
+  def tile_fence(self):
+    w = self.w
+    w('__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");')
+    w('__builtin_amdgcn_wave_barrier();')
+    w('__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");')
+
+  def t_rows(self, step, q, tile, pitch, js):
+    """Rows ``js`` of step (kb, t)'s temporary: FP64, from the lanes' rows via bpermute, rounded
+    or split into B tile ``tile``."""
+    w = self.w
+    kb, ti = step
     tabs = self.tables[ti]
-    w(f'// term {ti}: temporary rows (FP64) into the B tile')
-    w('#pragma unroll 4')
-    with w.block(f'for (int j = 0; j < {KB}; ++j)'):
-      w(f'const double qv = {self.qexpr};')
-      w('const int qlo = __double2loint(qv);')
-      w('const int qhi = __double2hiint(qv);')
-      w('double tv = 0.0;')
+    for j in js:
+      v = f'tv{ti}_{j}'
+      w(f'{{ const int qlo = __double2loint({q}[{j}]); const int qhi = __double2hiint({q}[{j}]);')
+      w(f'  double {v} = 0.0;')
       for si, (src, pv) in enumerate(tabs):
-        w(f'tv = __builtin_fma(__hiloint2double(__builtin_amdgcn_ds_bpermute({src}, qhi), '
-          f'__builtin_amdgcn_ds_bpermute({src}, qlo)), w{ti}_{si}, tv);')
-      if bf16:
-        w('const float tx = (float)tv;')
-        w('const unsigned h = tf_bf16(tx);')
-        w(f'th[gcol * {pitch} + j] = (unsigned short)h;')
-        w(f'tl[gcol * {pitch} + j] = (unsigned short)tf_bf16(tx - tf_bf16_f(h));')
+        w(f'  {v} = __builtin_fma(__hiloint2double(__builtin_amdgcn_ds_bpermute({src}, qhi), '
+          f'__builtin_amdgcn_ds_bpermute({src}, qlo)), w{ti}_{si}, {v});')
+      if self.mode == 'bf16x3':
+        w(f'  const float tx = (float){v}; const unsigned h = tf_bf16(tx);')
+        w(f'  th{tile}[gcol * {pitch} + {j}] = (unsigned short)h;')
+        w(f'  tl{tile}[gcol * {pitch} + {j}] = (unsigned short)tf_bf16(tx - tf_bf16_f(h)); }}')
       else:
-        w(f'tf[gcol * {pitch} + j] = (float)tv;')
-    w('__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");')
-    w('__builtin_amdgcn_wave_barrier();')
-    w('__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");')
+        w(f'  tf{tile}[gcol * {pitch} + {j}] = (float){v}; }}')
+
+  def mfma_list(self, kb, ti, tile, pitch):
+    """Source lines of step (kb, ti)'s MFMAs from B tile ``tile``: fragment loads, then the
+    MFMAs, as statements of the enclosing scope (names are unique per step)."""
+    RT, CT = self.RT, self.CT
     nks = KB // self.kstep
-    frag = f'((({ti} * {self.KS} + kb * {nks} + ks) * {RT} + rt) * 64 + lane)'
-    w('#pragma unroll')
-    with w.block(f'for (int ks = 0; ks < {nks}; ++ks)'):
-      if bf16:
-        w(f'tf_s4 bh[{CT}], bl[{CT}];')
-        w('#pragma unroll')
-        with w.block(f'for (int ct = 0; ct < {CT}; ++ct)'):
-          w(f'const int o = (ct * 16 + lane % 16) * {pitch} + 4 * (lane / 16);')
-          w('const uint2 vh = *reinterpret_cast<const uint2*>(th + o);')
-          w('const uint2 vl = *reinterpret_cast<const uint2*>(tl + o);')
-          w('bh[ct] = __builtin_bit_cast(tf_s4, vh);')
-          w('bl[ct] = __builtin_bit_cast(tf_s4, vl);')
-        w('#pragma unroll')
-        with w.block(f'for (int rt = 0; rt < {RT}; ++rt)'):
-          w(f'const tf_s4 ah = __builtin_bit_cast(tf_s4, kh[{frag}]);')
-          w(f'const tf_s4 al = __builtin_bit_cast(tf_s4, kl[{frag}]);')
-          w('#pragma unroll')
-          with w.block(f'for (int ct = 0; ct < {CT}; ++ct)'):
-            w(f'acc[rt * {CT} + ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[ct], '
-              f'acc[rt * {CT} + ct], 0, 0, 0);')
-            w(f'acr[rt * {CT} + ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[ct], '
-              f'acr[rt * {CT} + ct], 0, 0, 0);')
-            w(f'acr[rt * {CT} + ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[ct], '
-              f'acr[rt * {CT} + ct], 0, 0, 0);')
-      else:
-        w(f'float bv[{CT}];')
-        w('#pragma unroll')
-        with w.block(f'for (int ct = 0; ct < {CT}; ++ct)'):
-          w(f'bv[ct] = tf[(ct * 16 + lane % 16) * {pitch} + ks * 4 + lane / 16];')
-        w('#pragma unroll')
-        with w.block(f'for (int rt = 0; rt < {RT}; ++rt)'):
-          w(f'const float av = kf[{frag}];')
-          w('#pragma unroll')
-          with w.block(f'for (int ct = 0; ct < {CT}; ++ct)'):
-            w(f'acc[rt * {CT} + ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[ct], '
-              f'acc[rt * {CT} + ct], 0, 0, 0);')
-    # the next term rewrites the tile: its reads above are complete (consumed by the MFMAs)
-    w('__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");')
-    w('__builtin_amdgcn_wave_barrier();')
-    w('__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");')
+    out = []
+    sfx = f'_{kb}_{ti}'
+    for ks in range(nks):
+      for ct in range(CT):
+        if self.mode == 'bf16x3':
+          o = f'(({ct * 16} + lane % 16) * {pitch} + 4 * (lane / 16))'
+          out.append(f'const tf_s4 bh{ks}_{ct}{sfx} = __builtin_bit_cast(tf_s4, '
+                     f'*reinterpret_cast<const uint2*>(th{tile} + {o}));')
+          out.append(f'const tf_s4 bl{ks}_{ct}{sfx} = __builtin_bit_cast(tf_s4, '
+                     f'*reinterpret_cast<const uint2*>(tl{tile} + {o}));')
+        else:
+          out.append(f'const float bv{ks}_{ct}{sfx} = tf{tile}[({ct * 16} + lane % 16) * {pitch} + '
+                     f'{ks * 4} + lane / 16];')
+    for ks in range(nks):
+      for rt in range(RT):
+        frag = f'{((ti * self.KS + kb * nks + ks) * RT + rt) * 64} + lane'
+        if self.mode == 'bf16x3':
+          out.append(f'const tf_s4 ah{ks}_{rt}{sfx} = __builtin_bit_cast(tf_s4, kh[{frag}]);')
+          out.append(f'const tf_s4 al{ks}_{rt}{sfx} = __builtin_bit_cast(tf_s4, kl[{frag}]);')
+        else:
+          out.append(f'const float av{ks}_{rt}{sfx} = kf[{frag}];')
+        for ct in range(CT):
+          a = f'acc[{rt * CT + ct}]'
+          if self.mode == 'bf16x3':
+            r = f'acr[{rt * CT + ct}]'
+            out.append(f'{a} = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah{ks}_{rt}{sfx}, '
+                       f'bh{ks}_{ct}{sfx}, {a}, 0, 0, 0);')
+            out.append(f'{r} = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah{ks}_{rt}{sfx}, '
+                       f'bl{ks}_{ct}{sfx}, {r}, 0, 0, 0);')
+            out.append(f'{r} = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al{ks}_{rt}{sfx}, '
+                       f'bh{ks}_{ct}{sfx}, {r}, 0, 0, 0);')
+          else:
+            out.append(f'{a} = __builtin_amdgcn_mfma_f32_16x16x4f32(av{ks}_{rt}{sfx}, '
+                       f'bv{ks}_{ct}{sfx}, {a}, 0, 0, 0);')
+    return out
 
   def store(self):
     w = self.w
