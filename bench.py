@@ -34,7 +34,7 @@ def parse():
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--cpu-seconds', type=float, default=15.0)
   ap.add_argument('--only', default='',
-                  help='comma list of legs (tuned,plain,f32,copy,mixed,precision,tuner)')
+                  help='comma list of legs (tuned,plain,f32,copy,mixed,precision,flux,tuner)')
   return ap.parse_args()
 
 
@@ -246,6 +246,30 @@ def precision_leg(device, steps, warmup, n=262144):
   return res
 
 
+def flux_leg(device, steps, warmup, n=262144):
+  """Local and neighbour flux integrals (reference benchmarks/fused/programs/s3-*-integral.cf,
+  operand order 'mod'), FP64, tuned and plain launchers on the same inputs."""
+  import torch
+  from tensorforge_amd import catalog
+  from tensorforge_amd.tune.measure import synthetic_buffers
+  res = {}
+  for kind in ('local', 'neigh'):
+    entry, tuned = catalog.load(f'flux_{kind}_mod_f64_tuned')
+    _, plain = catalog.load(f'flux_{kind}_mod_f64_plain')
+    bufs = synthetic_buffers(tuned.generator, n, device)
+    bufs.pop('__keep__')
+    bpe = tuned.generator.get_emitted().bytes_per_element
+    t_t = time_kernel(tuned, bufs, n, steps, warmup)
+    t_p = time_kernel(plain, bufs, n, max(3, steps // 5), 2)
+    res[kind] = {'tuned': {'value': n / t_t, 'ms_per_step': t_t * 1e3, 'hbm_GBps': bpe * n / t_t / 1e9},
+                 'plain': {'value': n / t_p, 'ms_per_step': t_p * 1e3},
+                 'tuned_over_plain': t_p / t_t, 'bytes_per_element': bpe}
+    del bufs
+  res['unit'] = 'elements/s'
+  res['elements'] = n
+  return res
+
+
 def tuner_leg():
   """Cost model vs the exhaustive sweep: five-fold cross-validation by chain over the
   committed timing database (70 random chains, every applicable candidate timed on MI355X);
@@ -351,7 +375,7 @@ def main():
   from tensorforge_amd import catalog
   device = torch.device('cuda:0')
   legs = set(args.only.split(',')) if args.only else {'tuned', 'plain', 'f32', 'copy', 'mixed',
-                                                      'precision', 'tuner'}
+                                                      'precision', 'flux', 'tuner'}
   n = args.elements
   out = {}
 
@@ -412,6 +436,9 @@ def main():
 
   if 'precision' in legs:
     out['precision_order7'] = precision_leg(device, max(3, args.steps // 5), 2)
+
+  if 'flux' in legs:
+    out['flux'] = flux_leg(device, args.steps, args.warmup)
 
   if 'tuner' in legs:
     out['cost_model'] = tuner_leg()

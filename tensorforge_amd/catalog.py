@@ -180,6 +180,15 @@ for _emitter in ('plain', 'tuned'):
     register(f'ader_dense_{_fpn}_{_emitter}')(
       lambda fp=_fp, e=_emitter: _wl_entry(W.ader_volume(fp, sparse_k=False), e))
 
+# flux integrals either side of the volume chain (reference benchmarks/fused/programs)
+for _emitter in ('plain', 'tuned'):
+  for _ord in ('mod', 'orig'):
+    register(f'flux_local_{_ord}_f64_{_emitter}')(
+      lambda e=_emitter, o=_ord: _wl_entry(W.flux_local(F64, o), e))
+    register(f'flux_neigh_{_ord}_f64_{_emitter}')(
+      lambda e=_emitter, o=_ord: _wl_entry(W.flux_neigh(F64, o), e))
+
+
 # precision study: order 7 (84 basis functions), dense K_d, FP64 inputs and outputs
 for _emitter in ('plain', 'tuned', 'mfma_fp32', 'mfma_bf16x3'):
   register(f'ader7_f64_{_emitter}')(

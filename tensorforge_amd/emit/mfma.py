@@ -69,6 +69,7 @@ class _Mfma:
     self.kstep = 4 if mode == 'fp32' else 16
     self.KS = self.NKB * (KB // self.kstep)
     self.T = len(pl.terms)
+    self.prefetch_rows = mode != 'bf16x3'
 
   # --------------------------------------------------------------- per-lane star slots
   def slot_tables(self, t):
@@ -259,6 +260,8 @@ class _Mfma:
     # while the lanes form step s + 1's temporary rows into the other tile; the streamed rows of
     # block kb + 1 are loaded when block kb starts
     steps = [(kb, t) for kb in range(self.NKB) for t in range(self.T)]
+    # without prefetch the next block's rows are loaded one step ahead (fewer live registers)
+    nxt_loads = lambda si: (si + 1 < len(steps) and steps[si + 1][0] != steps[si][0])
     w(f'double qa[{KB}], qb[{KB}];')
     self.load_rows(0, 'qa')
     self.t_rows(steps[0], 'qa', 0, pitch, range(KB))
@@ -267,7 +270,9 @@ class _Mfma:
       qcur = 'qa' if kb % 2 == 0 else 'qb'
       qnxt = 'qb' if kb % 2 == 0 else 'qa'
       w(f'// step {si}: block {kb}, term {t}')
-      if t == 0 and kb + 1 < self.NKB:
+      if t == 0 and kb + 1 < self.NKB and self.prefetch_rows:
+        self.load_rows(kb + 1, qnxt)
+      if not self.prefetch_rows and nxt_loads(si):
         self.load_rows(kb + 1, qnxt)
       nxt = steps[si + 1] if si + 1 < len(steps) else None
       mf = self.mfma_list(kb, t, si % 2, pitch)

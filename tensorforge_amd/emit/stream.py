@@ -486,6 +486,8 @@ class _Stream:
     self._nxt_ptr_done = False
     if self.own_columns():
       w(f'const {pl.L.arg.dtype}* __restrict__ lp_cur = {self.lane_pointer("e0")};')
+    if self.row_loads():
+      self.row_offsets('e0', 'ro')
     for bi, b0 in enumerate(self.blocks):
       w(f'// block [{b0}, {min(b0 + self.KB, pl.khi)})')
       # issue the loads of the block D ahead (this group's, or the next group's)
@@ -526,6 +528,8 @@ class _Stream:
       if pl.L is not None:
         size = nrmax if self.own_columns() else self._pipe_steps(nrmax)
         w(f'{self.real} pf{j}[{size}];')
+    if self.row_loads():
+      self.row_offsets('efirst', 'ro')
       self.pipe_names.append(f'pf{j}')
     if self.own_columns():
       w(f'const int lane_ro = (slot * {self.l_q} + col) * {self.ps};')
@@ -640,7 +644,43 @@ class _Stream:
     """The streamed operand has as many columns as the output: lane (slot, col) loads the
     rows of its own element's column, so addresses are one per-lane pointer plus constants."""
     return self.pl.L is not None and self.l_q == self.q and \
-        self.pl.L.arg.addressing == Addressing.STRIDED and self.sched.stream_own_columns
+        self.pl.L.arg.addressing == Addressing.STRIDED and self.sched.stream_own_columns and \
+        not self.row_loads()
+
+  def row_loads(self):
+    """Consecutive lanes load consecutive rows of one column (KB rows x 8 columns per load
+    instruction for doubles), from a wavefront-uniform base and per-lane offsets computed once
+    per element group. Needs full blocks."""
+    pl = self.pl
+    return (pl.L is not None and pl.L.arg.addressing == Addressing.STRIDED and
+            self.sched.stream_row_loads and not self.loop and
+            (pl.khi - pl.klo) % self.KB == 0 and
+            not any(t.kind == 'global' for t in pl.terms))
+
+  def row_offsets(self, ebase, name):
+    """Declare ``{name}_q`` (wavefront-uniform base of the element group at ``ebase``, clamped
+    into the batch) and per-step lane offsets ``{name}{s}`` of the rows the lane loads."""
+    w = self.w
+    pl = self.pl
+    L = pl.L
+    a = L.arg
+    KB = self.KB
+    total = self.pairs * KB
+    w(f'const size_t {name}_e = {ebase} < numElements ? {ebase} : numElements - 1;')
+    w(f'const {a.dtype}* __restrict__ {name}_q = {a.name} + {a.name}_extraOffset + '
+      f'{name}_e * {a.element_stride}ull;')
+    for s_ in range(self._pipe_steps(KB)):
+      idx = f'(lane + {s_ * WAVE})' if s_ * WAVE + WAVE <= total else \
+          f'(lane + {s_ * WAVE} < {total} ? lane + {s_ * WAVE} : {total - 1})'
+      w(f'int {name}{s_};')
+      with w.block(''):
+        w(f'const int ix = {idx};')
+        w(f'const int pr = ix / {KB};')
+        w(f'const int el = pr / {self.l_q};')
+        w(f'const int cc = pr - el * {self.l_q};')
+        w(f'const int ev = ({name}_e + el < numElements ? el : (int)(numElements - 1 - {name}_e));')
+        w(f'{name}{s_} = ev * {a.element_stride} + (cc + {pl.l_cols[0] - L.origin[1]}) * '
+          f'{L.strides[1]} + (ix - pr * {KB} + {pl.klo - L.origin[0]}) * {L.strides[0]};')
 
   def lane_pointer(self, ebase):
     """Expression of the lane's pointer into its element's column of L for group ``ebase``."""
@@ -666,6 +706,17 @@ class _Stream:
         self._nxt_ptr_done = True
       for kk in range(nr):
         w(f'pf{reg}[{kk}] = ({self.real}){ptr}[{(b0 + kk - L.origin[0]) * L.strides[0]}];')
+    elif pl.L is not None and self.row_loads():
+      L = pl.L
+      if ebase == 'e0':
+        name = 'ro'
+      else:
+        if not getattr(self, '_nxt_ptr_done', False):
+          self.row_offsets('e0n', 'rn')
+          self._nxt_ptr_done = True
+        name = 'rn'
+      for s_ in range(self._pipe_steps(nr)):
+        w(f'pf{reg}[{s_}] = ({self.real}){name}_q[{name}{s_} + {(b0 - pl.klo) * L.strides[0]}];')
     elif pl.L is not None:
       L = pl.L
       total = self.pairs * nr

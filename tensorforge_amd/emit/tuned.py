@@ -86,6 +86,8 @@ class Schedule:
   stream_prefetch: int = 1
   # lane (element, column) streams its own column of the per-element operand
   stream_own_columns: bool = True
+  # lanes load consecutive rows of one column from a uniform base (coalesced row blocks)
+  stream_row_loads: bool = False
   # shared-operand chunks loaded ahead of use (0: loads placed by the compiler)
   stream_lds_lookahead: int = 2
   # rows of the streamed operand loaded straight into LDS (global_load_lds), ring of this many

@@ -16,6 +16,7 @@ import numpy as np
 
 from .common.aux import generate_tmp_tensor
 from .common.basic_types import Addressing, FloatingPointType
+from .common.matrix.boundingbox import BoundingBox
 from .common.matrix.spp import ListSPP
 from .common.matrix.tensor import SubTensor, Tensor
 from .generators.descriptions import MultilinearDescr
@@ -91,6 +92,75 @@ def ader_volume(fp_type=FloatingPointType.DOUBLE, b=56, q=9, sparse_k=True, dens
                                    [[0, 1], [0, 1]], add=d > 0))
   return Workload('ader_volume', descrs, fp_type, tensors,
                   {'b': b, 'q': q, 'sparse_k': sparse_k, 'masks': masks})
+
+
+def flux_local(fp_type=FloatingPointType.DOUBLE, ordering='mod') -> Workload:
+  """Local flux integral (reference ``benchmarks/fused/programs/s3-local-integral.cf``):
+  ``i_surf += r_div_m · (f_mr_t · (d_k · a_plus))`` with 64-row padded storage and bounding
+  boxes of 56 x 9, 56 x 21, 21 x 56."""
+  BB = BoundingBox
+  i_surf = Tensor([64, 9], Addressing.STRIDED, BB([0, 0], [56, 9]), alias='i_surf')
+  r_div_m = Tensor([64, 32], Addressing.NONE, BB([0, 0], [56, 21]), alias='r_div_m')
+  f_mr_t = Tensor([32, 64], Addressing.NONE, BB([0, 0], [21, 56]), alias='f_mr_t')
+  d_k = Tensor([64, 9], Addressing.STRIDED, BB([0, 0], [56, 9]), alias='d_k')
+  a_plus = Tensor([9, 9], Addressing.STRIDED, BB([0, 0], [9, 9]), alias='a_plus')
+  sub = lambda t: SubTensor(t, t.bbox)
+  pair = [[0, -1], [-1, 1]]
+  ident = [[0, 1], [0, 1]]
+  if ordering == 'mod':       # r_div_m · (f_mr_t · (d_k · a_plus))
+    t1 = generate_tmp_tensor([sub(d_k), sub(a_plus)], pair)
+    t2 = generate_tmp_tensor([sub(f_mr_t), SubTensor(t1)], pair)
+    descrs = [MultilinearDescr(SubTensor(t1), [sub(d_k), sub(a_plus)], pair, ident),
+              MultilinearDescr(SubTensor(t2), [sub(f_mr_t), SubTensor(t1)], pair, ident)]
+  else:                       # r_div_m · ((f_mr_t · d_k) · a_plus)
+    t1 = generate_tmp_tensor([sub(f_mr_t), sub(d_k)], pair)
+    t2 = generate_tmp_tensor([SubTensor(t1), sub(a_plus)], pair)
+    descrs = [MultilinearDescr(SubTensor(t1), [sub(f_mr_t), sub(d_k)], pair, ident),
+              MultilinearDescr(SubTensor(t2), [SubTensor(t1), sub(a_plus)], pair, ident)]
+  descrs.append(MultilinearDescr(sub(i_surf), [sub(r_div_m), SubTensor(t2)], pair, ident,
+                                 add=True))
+  tensors = {'i_surf': i_surf, 'r_div_m': r_div_m, 'f_mr_t': f_mr_t, 'd_k': d_k,
+             'a_plus': a_plus}
+  return Workload('flux_local', descrs, fp_type, tensors, {'ordering': ordering, 'b': 56, 'q': 9})
+
+
+def flux_neigh(fp_type=FloatingPointType.DOUBLE, ordering='mod') -> Workload:
+  """Neighbour flux integral (reference ``benchmarks/fused/programs/s3-neigh-integral.cf``):
+  ``dofs += r_div_mat · (f_plus · (r_mat · (i_dofs · a_plus)))``."""
+  BB = BoundingBox
+  dofs = Tensor([64, 9], Addressing.STRIDED, BB([0, 0], [56, 9]), alias='dofs')
+  r_div_mat = Tensor([64, 21], Addressing.NONE, BB([0, 0], [56, 21]), alias='r_div_mat')
+  f_plus = Tensor([32, 21], Addressing.NONE, BB([0, 0], [21, 21]), alias='f_plus')
+  r_mat = Tensor([32, 56], Addressing.NONE, BB([0, 0], [21, 56]), alias='r_mat')
+  i_dofs = Tensor([64, 9], Addressing.STRIDED, BB([0, 0], [56, 9]), alias='i_dofs')
+  a_plus = Tensor([9, 9], Addressing.STRIDED, BB([0, 0], [9, 9]), alias='a_plus')
+  sub = lambda t: SubTensor(t, t.bbox)
+  descrs = []
+  if ordering == 'mod':
+    t1 = generate_tmp_tensor([sub(i_dofs), sub(a_plus)], [[0, -1], [-1, 1]])
+    descrs.append(MultilinearDescr(SubTensor(t1), [sub(i_dofs), sub(a_plus)], [[0, -1], [-1, 1]],
+                                   [[0, 1], [0, 1]]))
+    t2 = generate_tmp_tensor([sub(r_mat), SubTensor(t1)], [[0, -1], [-1, 1]])
+    descrs.append(MultilinearDescr(SubTensor(t2), [sub(r_mat), SubTensor(t1)], [[0, -1], [-1, 1]],
+                                   [[0, 1], [0, 1]]))
+  else:
+    t0 = generate_tmp_tensor([sub(r_mat), sub(i_dofs)], [[0, -1], [-1, 1]])
+    descrs.append(MultilinearDescr(SubTensor(t0), [sub(r_mat), sub(i_dofs)], [[0, -1], [-1, 1]],
+                                   [[0, 1], [0, 1]]))
+    t2 = t0
+  t3 = generate_tmp_tensor([sub(f_plus), SubTensor(t2)], [[0, -1], [-1, 1]])
+  descrs.append(MultilinearDescr(SubTensor(t3), [sub(f_plus), SubTensor(t2)], [[0, -1], [-1, 1]],
+                                 [[0, 1], [0, 1]]))
+  if ordering == 'orig':
+    t4 = generate_tmp_tensor([SubTensor(t3), sub(a_plus)], [[0, -1], [-1, 1]])
+    descrs.append(MultilinearDescr(SubTensor(t4), [SubTensor(t3), sub(a_plus)], [[0, -1], [-1, 1]],
+                                   [[0, 1], [0, 1]]))
+    t3 = t4
+  descrs.append(MultilinearDescr(sub(dofs), [sub(r_div_mat), SubTensor(t3)], [[0, -1], [-1, 1]],
+                                 [[0, 1], [0, 1]], add=True))
+  tensors = {'dofs': dofs, 'r_div_mat': r_div_mat, 'f_plus': f_plus, 'r_mat': r_mat,
+             'i_dofs': i_dofs, 'a_plus': a_plus}
+  return Workload('flux_neigh', descrs, fp_type, tensors, {'ordering': ordering, 'b': 56, 'q': 9})
 
 
 # uneven batching: orders 2..7 (basis sizes) and their share of the elements
