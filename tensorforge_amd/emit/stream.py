@@ -59,6 +59,21 @@ class Plan:
   terms: List[Term]
   L: Optional[ir.Factor] = None  # streamed per-element operand of fused terms
   l_cols: tuple = ()             # column window of L
+  # register stages after the streamed one: D = U_n · ( ... U_2 · (streamed stage)); the streamed
+  # stage's accumulator is a temporary, the last stage writes the destination
+  post: list = field(default_factory=list)
+  final_init: str = 'zero'
+  final_beta: float = 0.0
+
+
+@dataclass
+class Post:
+  product: ir.Product
+  U: ir.Factor
+  u_dims: tuple
+  rows: tuple                    # destination rows of the stage
+  prev_lo: int                   # first row of the previous stage's accumulator
+  entries: list = field(default_factory=list)   # (row rel, prev row rel, position in U)
 
 
 def _classify(p):
@@ -68,6 +83,84 @@ def _classify(p):
 
 def plan(program: ir.ChainProgram):
   """Recognise the streaming form; returns (Plan, None) or (None, reason)."""
+  pl, why = _plan_single(program)
+  if pl is not None:
+    return pl, None
+  chain, why2 = _plan_chain(program)
+  if chain is not None:
+    return chain, None
+  return None, why
+
+
+def _plan_chain(program):
+  """``D (+)= U_n · (... (U_1 · (L · W)))``: shared left operands applied one after the other to
+  a temporary formed from per-element operands (SeisSol's flux integrals). The innermost
+  product streams like a single-stage chain; the outer ones run on the lane's registers."""
+  prods = program.products
+  consumers = [p for p in prods if p.dest_src == ir.SRC_GLOBAL]
+  if len(consumers) != 1:
+    return None, 'chain: not one destination product'
+  final = consumers[0]
+  by_dest = {id(q.dest): q for q in prods if q.dest_src == ir.SRC_TMP}
+  stages = []
+  cur = final
+  while True:
+    if cur.scalars:
+      return None, 'chain: scaled product'
+    sh, why = _classify(cur)
+    if sh is None:
+      return None, why
+    if sh.left.src != ir.SRC_GLOBAL or sh.left.arg.addressing != Addressing.NONE:
+      return None, 'chain: left operand is not shared'
+    if sh.right.src != ir.SRC_TMP or sh.right_dims != (0, 1):
+      return None, 'chain: right operand is not a temporary'
+    q = by_dest.get(id(sh.right.tensor))
+    if q is None or q.add or q.scalars:
+      return None, 'chain: temporary not produced once'
+    qsh, why = _classify(q)
+    if qsh is None:
+      return None, why
+    stages.append((cur, sh))
+    if qsh.left.src == ir.SRC_GLOBAL and qsh.left.arg.addressing == Addressing.NONE:
+      cur = q
+      continue
+    inner, inner_sh, producer, psh = cur, sh, q, qsh
+    break
+  if len(stages) < 2:
+    return None, 'chain: single stage'
+  stages.reverse()              # innermost (streamed) first
+  if len(stages) - 1 + 2 != len(prods):
+    return None, 'chain: other products'
+  if psh.left.src != ir.SRC_GLOBAL or not psh.left.arg.per_element or psh.left.is_sparse() or \
+      psh.left_dims != (0, 1) or psh.right.src != ir.SRC_GLOBAL or not psh.right.arg.per_element:
+    return None, 'chain: temporary not formed from per-element operands'
+  rows, cols = tuple(inner.n_ranges[0]), tuple(inner.n_ranges[1])
+  q = cols[1] - cols[0]
+  if not 1 <= q <= WAVE // 2:
+    return None, f'{q} columns'
+  klo, khi = inner.k_ranges[0]
+  if tuple(producer.n_ranges[0]) != (0, khi) or klo != 0 or producer.n_ranges[1][0] != 0 or \
+      producer.n_ranges[1][1] < q:
+    return None, 'chain: temporary rows or columns do not match'
+  post = []
+  prev_lo = rows[0]
+  for p, sh in stages[1:]:
+    if tuple(p.n_ranges[1]) != cols or p.n_ranges[0][1] - p.n_ranges[0][0] > 96:
+      return None, 'chain: stage columns or rows'
+    post.append(Post(p, sh.left, sh.left_dims, tuple(p.n_ranges[0]), prev_lo))
+    prev_lo = p.n_ranges[0][0]
+  if final.add and final.beta != 1.0:
+    return None, 'chain: scaled accumulation'
+  terms = [Term(inner, inner_sh.left, inner_sh.left_dims, 'fused', producer=producer,
+                W=psh.right, w_dims=psh.right_dims)]
+  return Plan(dest=final.dest_arg, dest_product=final, rows=rows, cols=cols, q=q, klo=klo,
+              khi=khi, init='zero', beta=0.0, terms=terms, L=psh.left,
+              l_cols=tuple(producer.k_ranges[0]), post=post,
+              final_init='load' if final.add else 'zero',
+              final_beta=final.beta if final.add else 0.0), None
+
+
+def _plan_single(program):
   prods = program.products
   consumers = [p for p in prods if p.dest_src == ir.SRC_GLOBAL]
   producers = [p for p in prods if p.dest_src == ir.SRC_TMP]
@@ -200,6 +293,22 @@ class _Stream:
         seq.append((b0, t_idx, e))
     self.blocks = blocks
     self.seq = seq
+    # register stages: entries by previous-stage row, then row, after the streamed entries
+    self.post_base = []
+    for st_idx, st in enumerate(pl.post):
+      U = st.U
+      nd, kd = st.u_dims
+      klo, khi = st.product.k_ranges[0]
+      if U.is_sparse():
+        items = U.packed.items()
+      else:
+        items = [(idx, U.offset(idx)) for idx in
+                 ir.iterate_box([(U.origin[d], U.origin[d] + U.tensor.shape[d]) for d in range(2)])]
+      st.entries = sorted((idx[kd] - st.prev_lo, idx[nd] - st.rows[0], pos) for idx, pos in items
+                          if st.rows[0] <= idx[nd] < st.rows[1] and klo <= idx[kd] < khi)
+      self.post_base.append(len(seq))
+      for j, i, pos in st.entries:
+        seq.append((None, len(pl.terms) + st_idx, (i, j, pos)))
     self.nu = len(seq)
     self.nu_pad = (self.nu + CHUNK - 1) // CHUNK * CHUNK
 
@@ -207,7 +316,7 @@ class _Stream:
     """Dense shared operands with many entries run the contraction blocks as a runtime loop
     (the fully unrolled body would not fit the instruction cache)."""
     pl = self.pl
-    if self.sched.stream_loop == 'never':
+    if self.sched.stream_loop == 'never' or pl.post:
       return False
     if any(t.U.is_sparse() for t in pl.terms):
       return False
@@ -293,9 +402,10 @@ class _Stream:
         w(f'const unsigned tv = {self.kname}_utab[e];')
         w('const unsigned t = tv >> 20;')
         w('const unsigned pos = tv & 0xfffffu;')
-        expr = f'({real}){pl.terms[-1].U.arg.name}[pos]'
-        for t_idx in range(len(pl.terms) - 2, -1, -1):
-          expr = f'(t == {t_idx} ? ({real}){pl.terms[t_idx].U.arg.name}[pos] : {expr})'
+        shared = [t.U for t in pl.terms] + [st.U for st in pl.post]
+        expr = f'({real}){shared[-1].arg.name}[pos]'
+        for t_idx in range(len(shared) - 2, -1, -1):
+          expr = f'(t == {t_idx} ? ({real}){shared[t_idx].arg.name}[pos] : {expr})'
         w(f'ubuf[e] = {expr};')
     elif self.u_source == 'lds':
       nt = len(pl.terms)
@@ -462,7 +572,7 @@ class _Stream:
     s0, s1 = dp.dest_strides
     for i in range(rows):
       off = f'{(pl.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
-      if pl.init == 'load':
+      if pl.init == 'load' and not pl.post:
         old = f'({real})g_{pl.dest.name}[{off}]'
         w(f'{acc}[{i}] = {old if pl.beta == 1.0 else f"{self.lit(pl.beta)} * {old}"};')
       else:
@@ -506,10 +616,76 @@ class _Stream:
       self.pipeline_write(nxt)
       self.pipeline_shift()
     self.pipeline_realign()
+    if pl.post:
+      self.post_stages(acc)
+      return
     with w.block('if (active)'):
       for i in range(rows):
         off = f'{(pl.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
         w(f'g_{pl.dest.name}[{off}] = {acc}[{i}];')
+
+  def post_stages(self, acc):
+    """Register stages: stage s accumulates U_s (LDS chunks, DPP broadcast) times the previous
+    stage's column; the last one starts from the destination when it accumulates and is
+    stored."""
+    w = self.w
+    pl = self.pl
+    real = self.real
+    prev = acc
+    ahead = max(1, self.sched.stream_lds_lookahead)
+    for s_idx, st in enumerate(pl.post):
+      last = s_idx == len(pl.post) - 1
+      name = f'acp{s_idx}'
+      nrows = st.rows[1] - st.rows[0]
+      dp = st.product
+      o0, o1 = dp.dest_origin
+      s0, s1 = dp.dest_strides
+      w(f'{real} {name}[{nrows}];')
+      for i in range(nrows):
+        if last and pl.final_init == 'load':
+          off = f'{(st.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
+          old = f'({real})g_{pl.dest.name}[{off}]'
+          w(f'{name}[{i}] = {old if pl.final_beta == 1.0 else f"{self.lit(pl.final_beta)} * {old}"};')
+        else:
+          w(f'{name}[{i}] = {self.lit(0)};')
+      base = self.post_base[s_idx]
+      order = []
+      for n in range(len(st.entries)):
+        c = (base + n) // CHUNK
+        if c not in order:
+          order.append(c)
+      cvar = {}
+      issued = [0]
+
+      def issue(upto):
+        while issued[0] < min(upto, len(order)):
+          c = order[issued[0]]
+          v = self.fresh('pc')
+          w(f'const {real} {v} = ubuf[{c * CHUNK} + lane15];')
+          cvar[c] = v
+          issued[0] += 1
+
+      w('asm volatile("" : "+v"(lane15));')
+      issue(ahead)
+      cur = None
+      for n, (j, i, pos) in enumerate(st.entries):
+        c, lane_in = divmod(base + n, CHUNK)
+        if c != cur:
+          if cur is not None:
+            w('__builtin_amdgcn_sched_barrier(0);')
+          issue(order.index(c) + 1 + ahead)
+          cur = c
+        self.fma(f'{name}[{i}]', cvar[c], lane_in, f'{prev}[{j}]')
+      w('__builtin_amdgcn_sched_barrier(0);')
+      prev = name
+    st = pl.post[-1]
+    dp = st.product
+    o0, o1 = dp.dest_origin
+    s0, s1 = dp.dest_strides
+    with w.block('if (active)'):
+      for i in range(st.rows[1] - st.rows[0]):
+        off = f'{(st.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
+        w(f'g_{pl.dest.name}[{off}] = {prev}[{i}];')
 
   # ---- cross-group prefetch pipeline (unrolled blocks) ----
   def _pipe_steps(self, nr):
