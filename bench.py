@@ -32,7 +32,7 @@ def parse():
   ap.add_argument('--warmup', type=int, default=5)
   ap.add_argument('--elements', type=int, default=262144)
   ap.add_argument('--no-cpu', action='store_true')
-  ap.add_argument('--cpu-seconds', type=float, default=15.0)
+  ap.add_argument('--cpu-seconds', type=float, default=25.0)
   ap.add_argument('--only', default='',
                   help='comma list of legs (tuned,plain,f32,copy,mixed,precision,flux,tuner)')
   return ap.parse_args()

@@ -70,3 +70,25 @@ def test_stream_direct_lds_ring_has_an_unaligned_fallback():
   # odd row counts cannot use 16-byte row pairs: no direct ring
   g5 = _gen(W.ader_volume(b=35, sparse_k=False), stream_dma=True, stream_loop='never')
   assert 'lds_dma_slots' not in g5.get_emitted().schedule
+
+
+def test_flux_chains_stream_with_register_stages():
+  """D += U2 · (U1 · (P · A)): the innermost product streams, the outer ones run on registers;
+  every stored entry of each shared matrix's window is one DPP FMA."""
+  for wl in (W.flux_local(), W.flux_neigh()):
+    g = _gen(wl)
+    em = g.get_emitted()
+    assert em.schedule['kind'] == 'stream'
+    pl, _ = stream.plan(g.get_program())
+    assert pl is not None and len(pl.post) >= 1
+    windows = {'local': [(21, 56), (56, 21)], 'neigh': [(21, 56), (21, 21), (56, 21)]}
+    kind = 'local' if 'i_surf' in wl.tensors else 'neigh'
+    expected = sum(r * c for r, c in windows[kind])
+    assert em.kernel_src.count('v_fmac_f64_dpp') == expected
+
+
+def test_row_bands_for_tall_dense_destinations():
+  g = _gen(W.ader_volume(b=84, sparse_k=False))
+  em = g.get_emitted()
+  assert em.schedule.get('row_passes') == 2 and em.schedule.get('loop')
+  assert em.lds_bytes <= 160 * 1024
