@@ -272,7 +272,7 @@ def flux_leg(device, steps, warmup, n=262144):
 
 def tuner_leg():
   """Cost model vs the exhaustive sweep: five-fold cross-validation by chain over the
-  committed timing database (70 random chains, every applicable candidate timed on MI355X);
+  committed timing database (218 random chains, every applicable candidate timed on MI355X);
   the network trains on the GPU in the project's HIP kernels."""
   from tensorforge_amd.tune import model as M
   path = os.path.join(ROOT, 'tensorforge_amd', 'tune', 'data', 'timings_mi355x.jsonl')

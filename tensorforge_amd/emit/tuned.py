@@ -26,6 +26,7 @@ import math
 from dataclasses import asdict, dataclass, field
 from typing import Dict, List, Optional, Tuple
 
+from ..common.exceptions import GenerationError
 from ..common.basic_types import Addressing, FloatingPointType
 from . import ir
 from .launcher import Emitted, Writer, kernel_params
@@ -845,7 +846,14 @@ def emit(program, context, base_name, schedule=None) -> Emitted:
       emitted = stream.emit(program, context, base_name, sched, pl)
       if emitted.lds_bytes <= LDS_LIMIT:
         return emitted
-  return _Emitter(program, context, base_name, schedule).emit()
+  emitted = _Emitter(program, context, base_name, schedule).emit()
+  if emitted.lds_bytes > LDS_LIMIT and schedule.waves_per_block > 1:
+    # fewer wavefronts per workgroup: each wavefront owns its LDS slice
+    emitted = _Emitter(program, context, base_name,
+                       Schedule(**{**asdict(schedule), 'waves_per_block': 1})).emit()
+  if emitted.lds_bytes > LDS_LIMIT:
+    raise GenerationError(f'tuned emitter: {emitted.lds_bytes} bytes of LDS per workgroup')
+  return emitted
 
 
 # LDS per workgroup on gfx950

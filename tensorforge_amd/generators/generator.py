@@ -158,8 +158,12 @@ class Generator:
     if choice in ('auto', 'tuned'):
       reason = tuned.unsupported_reason(self._program, self._context, self._schedule)
       if reason is None:
-        emitted = tuned.emit(self._program, self._context, self._base_kernel_name,
-                             self._schedule)
+        try:
+          emitted = tuned.emit(self._program, self._context, self._base_kernel_name,
+                               self._schedule)
+        except GenerationError:
+          if choice == 'tuned':
+            raise
       elif choice == 'tuned':
         raise GenerationError(f'tuned emitter cannot generate this chain: {reason}')
     if emitted is None:

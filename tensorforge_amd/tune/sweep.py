@@ -76,7 +76,11 @@ def measure(directory, steps=5, out_path=None):
         k = load_kernels(os.path.join(directory, cand['so']), [g])[0]
         if bufs is None:
           bufs = synthetic_buffers(g, batch, dev)
-        first = time_launcher(k, bufs, batch, steps=1, warmup=1)
+        try:
+          first = time_launcher(k, bufs, batch, steps=1, warmup=1)
+        except RuntimeError as err:   # the launcher reported a launch error: not a candidate
+          print(f'chain {rec["chain"]} {cand["label"]}: {err}', flush=True)
+          continue
         n_steps = int(min(steps, max(1, 0.02 / max(first, 1e-7))))
         ts = sorted(time_launcher(k, bufs, batch, steps=n_steps, warmup=0) for _ in range(3))
         out.write(json.dumps({'chain': rec['chain'], 'label': cand['label'], 'time': ts[1],

@@ -1,12 +1,13 @@
 """The learned cost model against the exhaustive sweep (GPU: the network trains in HIP).
 
-Data: ``tensorforge_amd/tune/data/timings_mi355x.jsonl``, every applicable candidate of 70
-random contraction chains timed on an MI355X (``python -m tensorforge_amd.tune.sweep``).
-Five-fold cross-validation by chain: each chain is judged by a network that never saw it.
+Data: ``tensorforge_amd/tune/data/timings_mi355x.jsonl``, every applicable candidate of 218
+random contraction chains timed on an MI355X (``python -m tensorforge_amd.tune.sweep``, two
+seeds). Five-fold cross-validation by chain: each chain is judged by a network that never saw it.
 
-What the network gains over the method without it: a search that times only the network's two
-best candidates plus the default choice lands within 5 % of the sweep's best on more chains, and
-slower on average by less, than the default choice alone, while timing 3 of ~10 candidates.
+What the network gains over the method without it (the library's default choice): its single
+pick is slower than the sweep's best by less on average, and a search that times only the
+network's two best candidates plus the default lands within 5 % of the sweep's best on most
+chains while timing 3 of ~9.4 candidates.
 """
 import os
 
@@ -24,9 +25,10 @@ def test_pruned_search_beats_the_default_choice(device):
   s = M.cross_validate(recs, folds=5, seed=0, prune_k=3, epochs=800, centered=True)
   print(s)
   cands = len(recs) / s['chains']
-  assert s['chains'] == 70 and cands > 9
-  assert s['pruned_search']['within_5pct'] >= 0.75
-  assert s['pruned_search']['within_5pct'] > s['default']['within_5pct'] + 0.05
-  assert s['pruned_search']['mean_slowdown'] < s['default']['mean_slowdown']
+  assert s['chains'] == 218 and cands > 9
+  assert s['model_pick']['mean_slowdown'] < s['default']['mean_slowdown']
+  assert s['pruned_search']['within_5pct'] >= 0.85
+  assert s['pruned_search']['within_5pct'] > s['default']['within_5pct'] + 0.1
+  assert s['pruned_search']['mean_slowdown'] < 0.5 * s['default']['mean_slowdown']
   # any single pick is far better than a random one (plain kernels are several times slower)
   assert s['model_pick']['mean_slowdown'] < 0.2 * s['random_mean']['mean_slowdown']
