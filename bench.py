@@ -367,8 +367,8 @@ def cpu_baseline(entry, bufs, gpu_out, budget_s):
 def main():
   args = parse()
   if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
-    print('bench.py measures one GPU: this build has no multi-GPU path (elements are independent; '
-          'see DESIGN.md for how the batch would be split). Run with --gpus 1.', file=sys.stderr)
+    print('bench.py measures one GPU: the launchers have no multi-GPU path (elements are independent; '
+          'a batch splits into contiguous per-GPU ranges). Run with --gpus 1.', file=sys.stderr)
     sys.exit(2)
   import numpy as np
   import torch

@@ -5,7 +5,7 @@ The model predicts log(time) of a (chain, batch, candidate) from generation-time
 
 * pick: the candidate with the smallest predicted time;
 * pruned search: time only the ``k - 1`` best-predicted candidates and the library's default
-  choice, keep the fastest measured one (the exhaustive sweep stays the fallback and the judge).
+  choice, keep the fastest measured one (the exhaustive sweep stays the fallback and the yardstick).
 
 ``evaluate`` reports, over chains held out of training, the slowdown of each method's choice
 against the sweep's best (``time(choice) / time(best) - 1``) for the model's pick, the pruned

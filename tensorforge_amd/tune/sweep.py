@@ -1,5 +1,5 @@
 """Exhaustive sweep of the candidate schedules over random contraction chains: the timing
-database the cost model is trained on, and the judge of its picks.
+database the cost model is trained on, and the yardstick for its picks.
 
     python -m tensorforge_amd.tune.sweep build   --dir D [--chains N] [--seed S]   (CPU: hipcc)
     python -m tensorforge_amd.tune.sweep measure --dir D                           (GPU)

@@ -2,7 +2,7 @@
 
 Data: ``tensorforge_amd/tune/data/timings_mi355x.jsonl``, every applicable candidate of 218
 random contraction chains timed on an MI355X (``python -m tensorforge_amd.tune.sweep``, two
-seeds). Five-fold cross-validation by chain: each chain is judged by a network that never saw it.
+seeds). Five-fold cross-validation by chain: each chain is scored by a network that never saw it.
 
 What the network gains over the method without it (the library's default choice): its single
 pick is slower than the sweep's best by less on average, and a search that times only the
