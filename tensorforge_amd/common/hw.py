@@ -35,7 +35,7 @@ def hw_descr_factory(arch, backend):
   """Return the hardware description; anything but gfx950/hip raises ``ValueError`` like the
   reference's factory does for an unknown architecture (``hw_descr.py:81``)."""
   if backend not in SUPPORTED_BACKENDS:
-    raise ValueError(f'Unknown backend, given: {backend} (this build emits gfx950 HIP only)')
+    raise ValueError(f'Unknown backend, given: {backend} (only gfx950 HIP is emitted)')
   if arch not in SUPPORTED_ARCHS:
-    raise ValueError(f'Unknown gpu architecture: {backend} {arch} (this build targets gfx950)')
+    raise ValueError(f'Unknown gpu architecture: {backend} {arch} (only gfx950 is targeted)')
   return SUPPORTED_ARCHS[arch]

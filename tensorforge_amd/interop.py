@@ -5,7 +5,7 @@ _HERE = os.path.dirname(os.path.realpath(__file__))
 
 
 def get_cmake_path():
-  """Directory of CMake package files. This build ships a ctypes/C-ABI library instead of a
+  """Directory of CMake package files. The package ships a ctypes/C-ABI library instead of a
   CMake package; the path names where ``tensorforge_aux.h`` and the C headers live."""
   return os.path.join(os.path.dirname(_HERE), 'include')
 
