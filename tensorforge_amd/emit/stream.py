@@ -336,6 +336,10 @@ class _Stream:
       self.KB = max(2, self.sched.stream_loop_k_block)
       self.order_entries()
       self.loop = self._loop_mode()
+    # shared entries read two chunks per instruction, streamed rows two per instruction
+    self.pair = bool(self.sched.stream_pair_reads) and not self.loop and self.real == 'double'
+    if self.pair:
+      self.nu_pad = (self.nu + 2 * CHUNK - 1) // (2 * CHUNK) * (2 * CHUNK)
     self.rows_pad = (pl.rows[1] - pl.rows[0] + CHUNK - 1) // CHUNK * CHUNK
     if self.loop:
       # shared entries laid out column by column, each (column, term) padded to whole chunks
@@ -354,7 +358,7 @@ class _Stream:
     if pl.L is not None:
       self.l_q = pl.l_cols[1] - pl.l_cols[0]
       self.pairs = G * self.l_q
-      self.ps = KB + 1
+      self.ps = KB + 2 if (self.pair and KB % 2 == 0) else KB + 1
       self.slot_words = self.pairs * self.ps
       ring = 2 * self.slot_words
       if self.allow_dma and self.dma_possible():
@@ -381,6 +385,10 @@ class _Stream:
       w(f'static __device__ const unsigned {self.kname}_utab[{max(1, self.nu)}] = '
         f'{{{tab or "0u"}}};')
       w('#endif')
+    w('#ifndef TF_D2_TYPE')
+    w('#define TF_D2_TYPE')
+    w('typedef double tf_d2 __attribute__((ext_vector_type(2)));')
+    w('#endif')
     params = ', '.join(kernel_params(self.prog, self.ctx))
     mw = max(1, self.sched.stream_min_waves)
     w(f'extern "C" __global__ void __launch_bounds__({threads}, {mw}) {self.kname}({params})')
@@ -406,7 +414,11 @@ class _Stream:
         expr = f'({real}){shared[-1].arg.name}[pos]'
         for t_idx in range(len(shared) - 2, -1, -1):
           expr = f'(t == {t_idx} ? ({real}){shared[t_idx].arg.name}[pos] : {expr})'
-        w(f'ubuf[e] = {expr};')
+        if self.pair:
+          # chunk pairs interleaved: entry j of chunks 2c and 2c + 1 side by side
+          w(f'ubuf[((e >> 5) << 5) | ((e & 15) << 1) | ((e >> 4) & 1)] = {expr};')
+        else:
+          w(f'ubuf[e] = {expr};')
     elif self.u_source == 'lds':
       nt = len(pl.terms)
       rp = self.rows_pad
@@ -660,9 +672,16 @@ class _Stream:
       def issue(upto):
         while issued[0] < min(upto, len(order)):
           c = order[issued[0]]
-          v = self.fresh('pc')
-          w(f'const {real} {v} = ubuf[{c * CHUNK} + lane15];')
-          cvar[c] = v
+          if c not in cvar:
+            if self.pair:
+              base = c - c % 2
+              v = self.fresh('pp')
+              w(f'const tf_d2 {v} = {self.chunk_pair_ref(base)};')
+              cvar[base], cvar[base + 1] = f'{v}.x', f'{v}.y'
+            else:
+              v = self.fresh('pc')
+              w(f'const {real} {v} = {self.chunk_ref(c)};')
+              cvar[c] = v
           issued[0] += 1
 
       w('asm volatile("" : "+v"(lane15));')
@@ -752,6 +771,16 @@ class _Stream:
   def dma_condition(self):
     a = self.pl.L.arg
     return f'(reinterpret_cast<uintptr_t>({a.name} + {a.name}_extraOffset) % 16u == 0u)'
+
+  def chunk_ref(self, c):
+    """LDS expression of lane15's entry of shared-operand chunk c."""
+    if self.pair:
+      return f'ubuf[{(c // 2) * 2 * CHUNK + c % 2} + lane15 * 2]'
+    return f'ubuf[{c * CHUNK} + lane15]'
+
+  def chunk_pair_ref(self, c):
+    """16-byte LDS expression of chunks (c, c + 1) for an even c (pair layout)."""
+    return f'*reinterpret_cast<const tf_d2*>(&ubuf[{(c // 2) * 2 * CHUNK} + lane15 * 2])'
 
   def ring_ref(self, ring_slot, o, kk):
     if self.dma:
@@ -1255,7 +1284,7 @@ class _Stream:
         c, j = divmod(n, CHUNK)
         if c not in chunks:
           cv = self.fresh('ch')
-          w(f'const {real} {cv} = ubuf[{c * CHUNK} + lane15];')
+          w(f'const {real} {cv} = {self.chunk_ref(c)};')
           chunks[c] = cv
         self.fma(f'{acc}[{i}]', chunks[c], j, rvals[ti])
         if i not in touched:
@@ -1297,20 +1326,39 @@ class _Stream:
     def issue(upto):
       while issued[0] < min(upto, len(order)):
         c = order[issued[0]]
-        v = self.fresh('ch')
-        w(f'const {real} {v} = ubuf[{c * CHUNK} + lane15];')
-        cvar[c] = v
+        if c not in cvar:
+          if self.pair:
+            base = c - c % 2
+            v = self.fresh('cp')
+            w(f'const tf_d2 {v} = {self.chunk_pair_ref(base)};')
+            cvar[base], cvar[base + 1] = f'{v}.x', f'{v}.y'
+          else:
+            v = self.fresh('ch')
+            w(f'const {real} {v} = {self.chunk_ref(c)};')
+            cvar[c] = v
         issued[0] += 1
+
+    pend = {}
+    pair_q = self.pair and pl.L is not None and self.ps % 2 == 0 and not self.dma
 
     def q_reads(k, ents):
       terms = set(s[1] for _, s in ents)
       if not any(pl.terms[ti].kind == 'fused' for ti in terms):
         return {}
       qv = {}
+      kk = k - b0
       for o in offs:
-        v = self.fresh('q')
-        w(f'const {real} {v} = {self.ring_ref(ring_slot, o, k - b0)};')
-        qv[o] = v
+        if pair_q and (o, kk) in pend:
+          qv[o] = pend.pop((o, kk))
+        elif pair_q and kk % 2 == 0 and k + 1 < b1:
+          v = self.fresh('qp')
+          w(f'const tf_d2 {v} = *reinterpret_cast<const tf_d2*>(&{self.ring_ref(ring_slot, o, kk)});')
+          qv[o] = f'{v}.x'
+          pend[(o, kk + 1)] = f'{v}.y'
+        else:
+          v = self.fresh('q')
+          w(f'const {real} {v} = {self.ring_ref(ring_slot, o, kk)};')
+          qv[o] = v
       return qv
 
     def t_rows(k, ents, qv):

@@ -89,6 +89,8 @@ class Schedule:
   stream_own_columns: bool = True
   # lanes load consecutive rows of one column from a uniform base (coalesced row blocks)
   stream_row_loads: bool = False
+  # two shared-operand chunks (and two streamed rows) per LDS read
+  stream_pair_reads: bool = False
   # shared-operand chunks loaded ahead of use (0: loads placed by the compiler)
   stream_lds_lookahead: int = 2
   # rows of the streamed operand loaded straight into LDS (global_load_lds), ring of this many
