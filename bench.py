@@ -128,7 +128,8 @@ def mixed_inputs(group, counts, device, seed=3):
 
 def mixed_leg(device, steps, warmup, graph_steps=10):
   """Uneven batch (orders 2..7, 500,000 elements, dense K_d, FP64): one step as plain
-  per-order launches, tuned per-order launches and one grouped persistent launch; and
+  per-order launches, tuned per-order launches (each order with its autotuned schedule,
+  tensorforge_amd/tune/data/mixed_orders_mi355x.json) and one grouped persistent launch; and
   ``graph_steps`` steps with Q and Q' swapping roles captured as one HIP graph (grouped launches,
   and tuned per-order launches)."""
   import torch

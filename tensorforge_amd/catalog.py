@@ -126,9 +126,12 @@ def get_group(name) -> GroupEntry:
   from .grouped import GroupedGenerator
   g = _GROUPS[name]()
   g.name = name
+  from .tune.orders import chosen_schedule
   for w in g.workloads:
     for emitter, dst in (('tuned', g.members), ('plain', g.plain)):
-      e = _wl_entry(w, emitter)
+      # tuned members use the autotuned schedule of their order (committed measurements)
+      sched = chosen_schedule(w.meta['b']) if emitter == 'tuned' else None
+      e = _wl_entry(w, emitter, sched)
       e.name = f'{name}_b{w.meta["b"]}_{emitter}'
       e.make_generator()
       e.generator.generate()

@@ -380,9 +380,11 @@ class _Stream:
     if not self.loop:
       tab = ', '.join(f'{(t_idx << 20) | e[2]}u' for (_, t_idx, e) in self.seq)
       # guarded: a grouped kernel repeats its members' tables in the same translation unit
-      w(f'#ifndef TF_{self.kname}_UTAB')
-      w(f'#define TF_{self.kname}_UTAB')
-      w(f'static __device__ const unsigned {self.kname}_utab[{max(1, self.nu)}] = '
+      # named per row band: the bands of one kernel have different tables
+      self.utab = f'{self.kname}_utab_r{pl.rows[0]}_{pl.rows[1]}'
+      w(f'#ifndef TF_{self.utab}')
+      w(f'#define TF_{self.utab}')
+      w(f'static __device__ const unsigned {self.utab}[{max(1, self.nu)}] = '
         f'{{{tab or "0u"}}};')
       w('#endif')
     w('#ifndef TF_D2_TYPE')
@@ -407,7 +409,7 @@ class _Stream:
     # stage the shared entries
     if not self.loop:
       with w.block(f'for (int e = threadIdx.x; e < {self.nu}; e += {threads})'):
-        w(f'const unsigned tv = {self.kname}_utab[e];')
+        w(f'const unsigned tv = {self.utab}[e];')
         w('const unsigned t = tv >> 20;')
         w('const unsigned pos = tv & 0xfffffu;')
         shared = [t.U for t in pl.terms] + [st.U for st in pl.post]
@@ -1449,7 +1451,13 @@ def emit(program, context, base_name, schedule, pl) -> Emitted:
   kname = first.kernel_name
   lines = first.kernel_src.rstrip('\n').split('\n')
   sig = next(i for i, l in enumerate(lines) if l.startswith('extern "C" __global__'))
-  head = lines[:sig + 2]
+  # every band's file-scope tables, then the kernel
+  head = []
+  for part in parts:
+    plines = part.kernel_src.rstrip('\n').split('\n')
+    psig = next(i for i, l in enumerate(plines) if l.startswith('extern "C" __global__'))
+    head += plines[:psig]
+  head += lines[sig:sig + 2]
   body = []
   for k, part in enumerate(parts):
     pl_lines = part.kernel_src.rstrip('\n').split('\n')

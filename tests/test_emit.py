@@ -1,4 +1,5 @@
 """Structure of the generated kernels (CPU: generation only)."""
+import re
 from tensorforge_amd import workloads as W
 from tensorforge_amd.common.basic_types import FloatingPointType
 from tensorforge_amd.common.context import Context
@@ -92,3 +93,15 @@ def test_row_bands_for_tall_dense_destinations():
   em = g.get_emitted()
   assert em.schedule.get('row_passes') == 2 and em.schedule.get('loop')
   assert em.lds_bytes <= 160 * 1024
+
+
+def test_row_bands_of_an_unrolled_kernel_have_their_own_tables():
+  """Each band stages its own rows of the shared matrices: a shared table name would make the
+  second band read the first band's entries."""
+  g = _gen(W.ader_volume(b=84, sparse_k=False), stream_loop='never', stream_pass_rows=48)
+  src = g.get_emitted().kernel_src
+  assert g.get_emitted().schedule.get('row_passes') == 2
+  names = set(re.findall(r'static __device__ const unsigned (\w+)\[', src))
+  assert len(names) == 2
+  for n in names:
+    assert f'{n}[e]' in src
