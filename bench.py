@@ -34,7 +34,8 @@ def parse():
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--cpu-seconds', type=float, default=25.0)
   ap.add_argument('--only', default='',
-                  help='comma list of legs (tuned,plain,f32,copy,mixed,precision,flux,tuner)')
+                  help='comma list of legs (tuned,plain,f32,copy,mixed,precision,flux,tuner,'
+                       'pointwise)')
   return ap.parse_args()
 
 
@@ -271,6 +272,35 @@ def flux_leg(device, steps, warmup, n=262144):
   return res
 
 
+def pointwise_leg(device, steps, warmup, n=262144):
+  """Pointwise kernels (ElementwiseDescr) at the headline element count: the update
+  Q += dt * I (FP64 and FP32, 2 reads + 1 write per entry) and the plastic return map on the six
+  stress columns (FP64; after the first launch every node is on or inside the yield surface, so
+  the timed launches store only the nodes that round outside it and ``hbm_GBps`` counts its
+  loads only). For the update ``hbm_GBps`` counts every entry's two loads and one store."""
+  from tensorforge_amd import catalog
+  from tensorforge_amd.tune.measure import synthetic_buffers
+  res = {}
+  for name in ('pw_update_f64', 'pw_update_f32', 'pw_plastic_f64'):
+    _, k = catalog.load(name)
+    bufs = synthetic_buffers(k.generator, n, device)
+    bufs.pop('__keep__')
+    if name.startswith('pw_plastic'):
+      bufs['cohesion'], bufs['tanphi'] = 1.2, 0.2
+    em = k.generator.get_emitted()
+    t = time_kernel(k, bufs, n, steps, warmup)
+    # the return map is idempotent: once every node is on or inside the surface, a launch
+    # stores (almost) nothing, so only its loads count
+    bpe = em.schedule['load_bytes_per_element'] if name.startswith('pw_plastic') else \
+        em.bytes_per_element
+    res[name] = {'value': n / t, 'ms_per_step': t * 1e3, 'bytes_per_element': bpe,
+                 'hbm_GBps': bpe * n / t / 1e9, 'frac_of_peak': bpe * n / t / 1e9 / HBM_PEAK_GBPS}
+    del bufs
+  res['unit'] = 'elements/s'
+  res['elements'] = n
+  return res
+
+
 def tuner_leg():
   """Cost model vs the exhaustive sweep: five-fold cross-validation by chain over the
   committed timing database (218 random chains, every applicable candidate timed on MI355X);
@@ -376,7 +406,7 @@ def main():
   from tensorforge_amd import catalog
   device = torch.device('cuda:0')
   legs = set(args.only.split(',')) if args.only else {'tuned', 'plain', 'f32', 'copy', 'mixed',
-                                                      'precision', 'flux', 'tuner'}
+                                                      'precision', 'flux', 'tuner', 'pointwise'}
   n = args.elements
   out = {}
 
@@ -440,6 +470,9 @@ def main():
 
   if 'flux' in legs:
     out['flux'] = flux_leg(device, args.steps, args.warmup)
+
+  if 'pointwise' in legs:
+    out['pointwise'] = pointwise_leg(device, args.steps, args.warmup)
 
   if 'tuner' in legs:
     out['cost_model'] = tuner_leg()

@@ -365,3 +365,66 @@ def _lane_wide_entry():
 
 
 _auto_variants()
+
+
+# ---- pointwise kernels (ElementwiseDescr) ----
+for _fp, _fpn in ((F64, 'f64'), (F32, 'f32')):
+  register(f'pw_update_{_fpn}')(lambda fp=_fp: _wl_entry(W.pointwise_update(fp), 'auto'))
+register('pw_plastic_f64')(lambda: _wl_entry(W.pointwise_plastic(F64), 'auto'))
+
+
+@register('t_pw_types')
+def _pw_types_entry():
+  """Pointwise coverage: pointer-based, shared (NONE) and constant-valued operands, a bounding
+  box inside a larger tensor with an extra offset, int and bool data, C++ integer division and
+  remainder, casts, a ternary, a while loop and floating math, FP64."""
+  from .generators import optree as ot
+  from .generators.descriptions import ElementwiseDescr
+  FT = FloatingPointType
+  A = _t([8, 3], Addressing.PTR_BASED, alias='A')
+  Wt = _t([8], Addressing.NONE, alias='W')
+  Cv = _t([8], Addressing.NONE, alias='Cv', data=[0.5, -1.0, 2.0, 0.0, 1.5, -0.25, 3.0, 1.0])
+  K = _t([8, 3], Addressing.STRIDED, alias='K', datatype=FT.INT)
+  D = _t([10, 5], Addressing.STRIDED, alias='D')
+  M = _t([8, 3], Addressing.STRIDED, alias='M', datatype=FT.INT)
+  s = _t([], Addressing.SCALAR, alias='s')
+  a = ot.tensor(A, indices=[-1, -2])
+  k = ot.tensor(K, indices=[-1, -2])
+  w = ot.tensor(Wt, indices=[-1])
+  cv = ot.tensor(Cv, indices=[-1])
+  d = ot.tensor(SubTensor(D, BoundingBox([1, 1], [9, 4])), indices=[-1, -2])
+  m = ot.tensor(M, indices=[-1, -2])
+  x, t, n = ot.temp(), ot.temp(), ot.temp(FT.INT)
+  stmts = [
+    ot.assign(x, abs(a) + 0.2),
+    ot.assign(t, 0.1),
+    ot.assign(n, 0),
+    ot.loop(t < x, [ot.assign(t, t * 2.0), ot.assign(n, n + 1)]),
+    ot.assign(d, t + ot.ternary(a > 0.0, w, cv) * ot.scalar(s) + ot.max(a, -0.5) ** 3.0
+              + ot.exp(ot.min(a, 0.0)) + ot.logp1(abs(a)) + ot.round(a * 4.0) + ot.floor(a * 3.0)),
+    ot.assign(m, k / 3 + (k % 3) * 10 + ot.cast(a > 0.0, FT.INT) * 100 + n * 1000
+              + ot.cast((k > 0) & (a < 0.5), FT.INT) * 10000),
+    ot.conditional(ot.bitnot(a > -0.5), [ot.assign(d, -d)]),
+  ]
+  return Entry('', [ElementwiseDescr(stmts)], F64,
+               {'A': A, 'W': Wt, 'Cv': Cv, 'K': K, 'D': D, 'M': M, 's': s})
+
+
+@register('t_pw_math_f32')
+def _pw_math_entry():
+  """Pointwise floating math in FP32: circular, hyperbolic (distinct from the circular ones),
+  inverse, power forms rewritten at construction (x^-0.5, x^(1/3)), erf and expm1."""
+  from .generators import optree as ot
+  from .generators.descriptions import ElementwiseDescr
+  X = _t([20, 9], Addressing.STRIDED, alias='X')
+  Y = _t([20, 9], Addressing.STRIDED, alias='Y')
+  x = ot.tensor(X, indices=[-1, -2])
+  y = ot.tensor(Y, indices=[-1, -2])
+  u = ot.temp()
+  stmts = [
+    ot.assign(u, abs(x) + 0.5),
+    ot.assign(y, ot.sin(x) + ot.cos(x) * 0.5 + ot.tanh(x) + ot.sinh(x) * 0.25 + u ** -0.5
+              + u ** (1 / 3) + ot.atan(x) + ot.erf(x) + ot.expm1(x * 0.5) + ot.log(u)
+              + ot.pow(u, 1.7) + ot.sqrt(u) / (u + 1.0) + ot.asinh(x) + ot.cosh(x) * 0.1),
+  ]
+  return Entry('', [ElementwiseDescr(stmts)], F32, {'X': X, 'Y': Y})

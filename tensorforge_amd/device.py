@@ -31,7 +31,9 @@ def launch_args(generator, buffers: Dict, num_elements: int, offsets: Optional[D
       args.append(float(tensor.data[0]) if not isinstance(tensor.data, dict) else 0.0)
       continue
     if tensor.addressing == Addressing.SCALAR:
-      args.append(float(buffers[key]))
+      integral = tensor.datatype is not None and tensor.datatype.name in (
+        'BOOL', 'INT', 'UINT', 'LONG', 'ULONG')
+      args.append(int(buffers[key]) if integral else float(buffers[key]))
       continue
     if tensor.has_values() and len(tensor.get_values()) < 16:
       args.append(0)   # baked constant: the pointer is not read

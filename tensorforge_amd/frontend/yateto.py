@@ -11,7 +11,7 @@ from ..common.context import Context
 from ..common.matrix.boundingbox import BoundingBox as BBox
 from ..common.matrix.spp import FullSPP, ListSPP
 from ..common.matrix.tensor import SubTensor, Tensor
-from ..generators.descriptions import MultilinearDescr
+from ..generators.descriptions import ElementwiseDescr, MultilinearDescr
 from ..generators.generator import Generator as TensorForgeGenerator
 from ..interface import YatetoInterface
 
@@ -21,6 +21,11 @@ _ADDRESSING = {'&': Addressing.NONE, 'n*N+o&': Addressing.STRIDED, 'n&+o&': Addr
 
 def _storage(layout):
   return layout.storage() if type(layout).__name__ == 'MemoryLayoutView' else layout
+
+
+def _name_of(op):
+  """yateto scalars and tensors expose ``name`` as a method, operation terms as an attribute."""
+  return op.name() if callable(op.name) else op.name
 
 
 class GpuKernelGeneratorV1:
@@ -38,6 +43,35 @@ class GpuKernelGeneratorV1:
       self.get_tensor(dest, aligned, list(range(len(dest.indices)))),
       [self.get_tensor(op, aligned, optarget) for op, optarget in zip(ops, target)],
       target, permute, add=add, strict_match=False, prefer_align=aligned))
+    return 0
+
+  # ---- pointwise statements ----
+  def add_scalar(self, ops, statements, indices):
+    """Pointwise statements (``optree``) over yateto tensors and scalars (reference
+    ``frontend/yateto.py:127-149``): ``indices[i]`` holds the index ids of ``ops[i]``; every
+    ``TensorVar``/``ScalarVar`` in ``statements`` is bound to its tensor by ``assignTensor``, a
+    tensor at the bounding box of its nonzeros (``eqspp.nnzbounds()``)."""
+    ids = {}
+    for op, idx in zip(ops, indices):
+      if _name_of(op) not in self._cache:      # records from add_tensor are already there
+        self.make_tensor(op, False, None)
+      ids[_name_of(op)] = idx
+
+    def assigner(pre):
+      name = _name_of(pre)
+      if self.is_scalar(pre) and not hasattr(pre, 'eqspp'):
+        if name not in self._cache:
+          self.make_tensor(pre, False, None)
+        ids[name] = []
+        return SubTensor(self._cache[name], BBox([], [])), []
+      eqspp = pre.eqspp() if callable(pre.eqspp) else pre.eqspp
+      bounds = eqspp.nnzbounds()
+      return (SubTensor(self._cache[name], BBox([lo for lo, _ in bounds], [hi + 1 for _, hi in bounds])),
+              ids[name])
+
+    for statement in statements:
+      statement.assignTensor(assigner)
+    self._descr_list.append(ElementwiseDescr(statements))
     return 0
 
   # ---- dictionary gateway ----
@@ -169,15 +203,27 @@ class GpuKernelGeneratorV1:
 
   # ---- code generation ----
   def generate(self, cpp, routine_cache):
+    """One kernel and one launcher call per run of descriptions of one kind: contraction chains
+    (``MultilinearDescr``) and pointwise statements (``ElementwiseDescr``) become separate
+    kernels launched in order on the same stream (the reference puts both in one kernel). A chain
+    temporary lives inside one kernel: using it across the split raises at generation."""
     fptype = FloatingPointType.str2enum(self._arch.typename) if hasattr(self._arch, 'typename') \
       else FloatingPointType.FLOAT
-    context = Context(arch=self._arch.name, backend=self._arch.backend, fp_type=fptype)
-    generator = TensorForgeGenerator(self._descr_list, context)
-    generator.register()
-    cpp(self._gen_call_site(generator))
-    name = generator.get_base_name()
-    generator.generate()
-    routine_cache.addRoutine(name, TensorForgeWriter(generator, context.get_vm().get_headers()))
+    segments = []
+    for d in self._descr_list:
+      pointwise = isinstance(d, ElementwiseDescr)
+      if segments and segments[-1][0] == pointwise:
+        segments[-1][1].append(d)
+      else:
+        segments.append((pointwise, [d]))
+    for _, descrs in segments:
+      context = Context(arch=self._arch.name, backend=self._arch.backend, fp_type=fptype)
+      generator = TensorForgeGenerator(descrs, context)
+      generator.register()
+      cpp(self._gen_call_site(generator))
+      name = generator.get_base_name()
+      generator.generate()
+      routine_cache.addRoutine(name, TensorForgeWriter(generator, context.get_vm().get_headers()))
 
   def _gen_call_site(self, generator):
     mat_name_map, offset_name_map = {}, {}
@@ -239,3 +285,6 @@ class YatetoFrontend:
 
   def add_tensor(self, tensor):
     return self.generator.add_tensor(tensor)
+
+  def add_scalar(self, ops, statements, indices):
+    return self.generator.add_scalar(ops, statements, indices)

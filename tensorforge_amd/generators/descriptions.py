@@ -96,12 +96,30 @@ class MultilinearDescr(OperationDescription):
 
 
 class ElementwiseDescr(OperationDescription):
-  """Pointwise assignments (reference ``descriptions.py:67-95``). Accepted by the API so lists
-  that contain them can be named, but the gfx950 emitter does not generate them."""
+  """Pointwise statements over every index point of every element (reference
+  ``descriptions.py:67-95``): ``oplist`` is a list of ``optree`` statements (``Assignment``,
+  ``IfNode``, ``WhileNode``). Tensors assigned to become sinks, tensors read become sources
+  (both: source-sink). Temporaries without a name are numbered ``tmp0, tmp1, ...`` in
+  first-appearance order, so ``str()`` (and the kernel name) is deterministic."""
 
   def __init__(self, oplist, strict_match: bool = False, prefer_align: bool = False):
-    self.oplist = oplist
+    from . import optree
+    self.oplist = list(oplist)
     self._strict_match = False
+    for op in self.oplist:
+      if not isinstance(op, optree.Statement):
+        raise GenerationError(f'ElementwiseDescr takes optree statements, got {type(op).__name__}')
+      for sub in op.tensors(outtensors=True, intensors=False):
+        sub.tensor.set_data_flow_direction(DataFlowDirection.SINK)
+      for sub in op.tensors(outtensors=False, intensors=True):
+        sub.tensor.set_data_flow_direction(DataFlowDirection.SOURCE)
+    count = 0
+    for s in optree.walk_statements(self.oplist):
+      for root in optree.statement_nodes(s):
+        for node in optree.walk_nodes(root):
+          if isinstance(node, optree.TempVar) and node.name is None:
+            node.name = f'tmp{count}'
+            count += 1
 
   def get_num_threads(self, context: Context):
     return 64, 64

@@ -141,12 +141,25 @@ class Generator:
   def generate(self):
     if not self._is_registerd:
       self.register()
-    for descr in self.descr_list:
-      if isinstance(descr, ElementwiseDescr):
-        raise GenerationError('ElementwiseDescr is not generated by the gfx950 emitter')
+    pointwise = [isinstance(d, ElementwiseDescr) for d in self.descr_list]
+    if any(pointwise) and not all(pointwise):
+      raise GenerationError('a kernel is either pointwise (ElementwiseDescr only) or a '
+                            'contraction chain (MultilinearDescr only); generate them as two '
+                            'kernels')
     self._deduce_num_threads()
-    self._program = ir_mod.lower(self.descr_list, self._global_tensors, self._context.fp_type)
     from ..emit import plain, tuned, launcher
+    if pointwise and all(pointwise):
+      from ..emit import pointwise as pw
+      if self._emitter_choice not in ('auto', 'tuned', 'plain'):
+        raise GenerationError(f'{self._emitter_choice} does not generate pointwise kernels')
+      self._program = pw.lower(self.descr_list, self._global_tensors, self._context.fp_type)
+      emitted = pw.emit(self._program, self._context, self._base_kernel_name)
+      self._emitted = emitted
+      self._kernel = emitted.full_source()
+      self._launcher = launcher.launcher_source(self, emitted)
+      self._header = f'{self._generate_launcher_proto(with_defaults=True)};\n'
+      return
+    self._program = ir_mod.lower(self.descr_list, self._global_tensors, self._context.fp_type)
     choice = self._emitter_choice
     emitted = None
     if choice in ('mfma_fp32', 'mfma_bf16x3'):

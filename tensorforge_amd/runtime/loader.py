@@ -46,6 +46,7 @@ def runtime():
 _CTYPES = {
   'double': ctypes.c_double, 'float': ctypes.c_float, 'unsigned': ctypes.c_uint,
   'size_t': ctypes.c_size_t, 'int': ctypes.c_int, 'long': ctypes.c_long,
+  'bool': ctypes.c_bool, 'unsigned int': ctypes.c_uint, 'unsigned long': ctypes.c_ulong,
 }
 
 

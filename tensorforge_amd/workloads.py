@@ -1,6 +1,12 @@
 """Description lists and seeded inputs of the benchmark workloads.
 
 * ``gemm``: ``C_e += A · B_e``, A b×b shared, B_e and C_e b×q per element (single batched GEMM).
+* ``pointwise_update``: ``Q_e += dt · I_e`` entry by entry (the ADER update of the degrees of
+  freedom with the time-integrated ones), ``dt`` a by-value scalar.
+* ``pointwise_plastic``: a nodal Drucker-Prager return map on the six stress columns of a b×9
+  nodal array: mean stress ``m``, deviatoric invariant ``tau = sqrt(J2)``, yield stress
+  ``c - m·tanphi``; nodes with ``tau`` above it have their deviator scaled back to the yield
+  surface.
 * ``ader_volume``: ``Q'_e = Σ_d K_d · Q_e · S_{e,d}``, d = 1..3: K_d b×b shared (sparse in CSR
   value order, or dense), Q_e b×q, S_{e,d} a q×q per-element star matrix with a fixed 24-entry
   pattern. This is the volume chain of SeisSol's ADER-DG step as written in the reference's
@@ -19,7 +25,8 @@ from .common.basic_types import Addressing, FloatingPointType
 from .common.matrix.boundingbox import BoundingBox
 from .common.matrix.spp import ListSPP
 from .common.matrix.tensor import SubTensor, Tensor
-from .generators.descriptions import MultilinearDescr
+from .generators import optree as ot
+from .generators.descriptions import ElementwiseDescr, MultilinearDescr
 
 # Elastic-wave star matrix pattern (quantities σxx σyy σzz σxy σyz σxz u v w = 0..8): the
 # stress rows couple to velocities, the velocity rows to stresses.
@@ -236,3 +243,33 @@ def launcher_args_by_alias(generator):
   """Launcher operands in order as aliases (``None`` for non-tensor arguments)."""
   return [(name, next((t.alias for t in generator._global_tensors if t.name == name.split('_')[0]), None))
           for _, name in generator.launcher_signature()]
+
+
+def pointwise_update(fp_type=FloatingPointType.DOUBLE, b=56, q=9) -> Workload:
+  Q = Tensor([b, q], Addressing.STRIDED, alias='Q')
+  I = Tensor([b, q], Addressing.STRIDED, alias='I')
+  dt = Tensor([], Addressing.SCALAR, alias='dt')
+  q_var = ot.tensor(Q, indices=[-1, -2])
+  stmt = ot.assign(q_var, q_var + ot.scalar(dt) * ot.tensor(I, indices=[-1, -2]))
+  return Workload('pointwise_update', [ElementwiseDescr([stmt])], fp_type,
+                  {'Q': Q, 'I': I, 'dt': dt}, {'b': b, 'q': q})
+
+
+def pointwise_plastic(fp_type=FloatingPointType.DOUBLE, b=56, q=9) -> Workload:
+  Q = Tensor([b, q], Addressing.STRIDED, alias='Q')
+  cohesion = Tensor([], Addressing.SCALAR, alias='cohesion')
+  tanphi = Tensor([], Addressing.SCALAR, alias='tanphi')
+  s = [ot.tensor(SubTensor(Q, BoundingBox([0, k], [b, k + 1])), indices=[-1, -2])
+       for k in range(6)]
+  m, tau, yld, f = ot.temp(), ot.temp(), ot.temp(), ot.temp()
+  stmts = [
+    ot.assign(m, (s[0] + s[1] + s[2]) / 3.0),
+    ot.assign(tau, ot.sqrt(0.5 * ((s[0] - m) ** 2 + (s[1] - m) ** 2 + (s[2] - m) ** 2)
+                           + s[3] ** 2 + s[4] ** 2 + s[5] ** 2)),
+    ot.assign(yld, ot.scalar(cohesion) - m * ot.scalar(tanphi)),
+    ot.conditional(tau > yld, [ot.assign(f, yld / tau)] +
+                   [ot.assign(s[k], m + f * (s[k] - m)) for k in range(3)] +
+                   [ot.assign(s[k], f * s[k]) for k in range(3, 6)]),
+  ]
+  return Workload('pointwise_plastic', [ElementwiseDescr(stmts)], fp_type,
+                  {'Q': Q, 'cohesion': cohesion, 'tanphi': tanphi}, {'b': b, 'q': q})

@@ -16,32 +16,60 @@ def np_dtype(fp):
   return np.float64 if fp == FloatingPointType.DOUBLE else np.float32
 
 
+_INT_DTYPES = {'INT': np.int32, 'UINT': np.uint32, 'LONG': np.int64, 'ULONG': np.uint64,
+               'BOOL': np.bool_}
+
+
+def tensor_dtype(t, fp):
+  """numpy dtype of a tensor's entries: its datatype, else the kernel's floating type."""
+  if t.datatype is None:
+    return np_dtype(fp)
+  if t.datatype.name in _INT_DTYPES:
+    return _INT_DTYPES[t.datatype.name]
+  return np_dtype(t.datatype)
+
+
+def _draw(rng, size, dt):
+  if dt == np.bool_:
+    return rng.random(size) < 0.5
+  if np.issubdtype(dt, np.integer):
+    lo = 0 if np.issubdtype(dt, np.unsignedinteger) else -20
+    return rng.integers(lo, 20, size=size).astype(dt)
+  return rng.uniform(-1, 1, size=size).astype(dt)
+
+
 def make_inputs(entry, num_elements, seed=0, offsets=None, garbage_unused=True):
   """Host inputs per tensor alias, in the launcher's layout.
 
-  Unused slots of packed sparse tensors hold random values so a kernel that reads them fails."""
+  Unused slots of packed sparse tensors hold random values so a kernel that reads them fails.
+  Integer tensors hold integers in [-20, 20), bool tensors random truth values."""
   rng = np.random.default_rng(seed)
   gen = entry.generator
-  dt = np_dtype(entry.fp_type)
   offsets = dict(offsets or {})
   host = {}
   for t in gen._global_tensors:
     key = key_of(t)
     vol = t.get_real_volume()
+    dt = tensor_dtype(t, entry.fp_type)
     if t.addressing == Addressing.SCALAR:
       if t.data is None:
-        host[key] = float(rng.uniform(0.5, 2.0))
+        host[key] = int(rng.integers(1, 5)) if np.issubdtype(dt, np.integer) else \
+          float(rng.uniform(0.5, 2.0))
       continue
     if t.has_values() and len(t.get_values()) < 16:
       continue
     off = int(offsets.get(key, 0))
     if t.addressing == Addressing.NONE:
-      host[key] = rng.uniform(-1, 1, size=vol).astype(dt)
+      host[key] = _draw(rng, vol, dt)
     elif t.addressing == Addressing.STRIDED:
-      host[key] = rng.uniform(-1, 1, size=num_elements * vol + off).astype(dt)
+      host[key] = _draw(rng, num_elements * vol + off, dt)
     else:
-      host[key] = [rng.uniform(-1, 1, size=vol + off).astype(dt) for _ in range(num_elements)]
+      host[key] = [_draw(rng, vol + off, dt) for _ in range(num_elements)]
   return host
+
+
+def is_pointwise(entry):
+  return any(type(d).__name__ == 'ElementwiseDescr' for d in entry.descrs)
 
 
 def oracle_outputs(entry, host, num_elements, offsets=None, flags=None):
@@ -51,10 +79,14 @@ def oracle_outputs(entry, host, num_elements, offsets=None, flags=None):
     if key_of(t) in host:
       v = host[key_of(t)]
       arrays[t] = [x.astype(np.float64).copy() for x in v] if isinstance(v, list) else \
-        (np.asarray(v, dtype=np.float64).copy() if not isinstance(v, float) else v)
+        (np.asarray(v, dtype=np.float64).copy() if not isinstance(v, (float, int)) else v)
   offs = {t: (offsets or {}).get(key_of(t), 0) for t in gen._global_tensors}
   inp = OracleInputs(arrays, num_elements, flags=flags, offsets=offs)
-  out = oracle_run(entry.descrs, inp)
+  if is_pointwise(entry):
+    from oracle.pointwise_oracle import run as pointwise_run
+    out = pointwise_run(entry.descrs, inp)
+  else:
+    out = oracle_run(entry.descrs, inp)
   return {key_of(t): out[t] for t in gen._global_tensors if t in out}
 
 
@@ -75,7 +107,7 @@ def gpu_run(kernel, entry, host, num_elements, device, offsets=None, flags=None)
   for key, v in host.items():
     if isinstance(v, list):
       bufs[key] = [torch.from_numpy(x.copy()).to(device) for x in v]
-    elif isinstance(v, float):
+    elif isinstance(v, (float, int)):
       bufs[key] = v
     else:
       bufs[key] = torch.from_numpy(v.copy()).to(device)
@@ -87,7 +119,7 @@ def gpu_run(kernel, entry, host, num_elements, device, offsets=None, flags=None)
   for key, v in bufs.items():
     if isinstance(v, list):
       out[key] = [x.cpu().numpy() for x in v]
-    elif not isinstance(v, float):
+    elif not isinstance(v, (float, int)):
       out[key] = v.cpu().numpy()
   return out
 

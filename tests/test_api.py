@@ -199,3 +199,67 @@ def test_yateto_frontend_dictionary_gateway():
   name = next(iter(routines))
   assert lines[0] == (f'launcher_{name}(C, extraOffset_C, const_cast<const double*>(A), '
                       'const_cast<const double*>(B), extraOffset_B, numElements, flags, streamPtr);')
+
+
+def test_yateto_frontend_pointwise_statements():
+  """add_scalar binds optree statements to yateto tensors and scalars through assignTensor
+  (reference frontend/yateto.py:127-149); a chain and pointwise statements become two kernels,
+  called in order."""
+  import tensorforge
+  from tensorforge.generators import optree as ot
+  fe_cls = tensorforge.get_routine_generator(None)
+
+  class Arch:
+    name, backend, typename = 'gfx950', 'hip', 'double'
+
+  class Spp:
+    def __init__(self, shape):
+      self.shape = shape
+
+    def nnzbounds(self):
+      return [(0, s - 1) for s in self.shape]
+
+  class YTensor:
+    def __init__(self, name, shape):
+      self._name, self._spp = name, Spp(shape)
+
+    def name(self):
+      return self._name
+
+    def eqspp(self):
+      return self._spp
+
+  class YScalar:
+    datatype = None
+
+    def __init__(self, name):
+      self._name = name
+
+    def name(self):
+      return self._name
+
+  fe = fe_cls(Arch())
+  for name, shape, addr in (('A', [8, 8], '&'), ('B', [8, 3], 'n*N+o&'), ('C', [8, 3], 'n*N+o&')):
+    fe.add_tensor({'name': name, 'datatype': 'f64', 'addressing': addr,
+                   'storage': {'shape': shape, 'type': 'full'}, 'values': None,
+                   'flags': {'temporary': False, 'constant': False}})
+  ref = lambda n: {'name': n, 'spp': None}
+  fe.add_operation({'type': 'multilinear', 'result': ref('C'), 'args': [ref('A'), ref('B')],
+                    'condition': [], 'target': [[0, -1], [-1, 1]], 'permute': [[0, 1], [0, 1]],
+                    'linear': {'alpha': None, 'add': False}})
+  C, dt = YTensor('C', [8, 3]), YScalar('dt')
+  c = ot.tensor(C)
+  fe.add_scalar([C], [ot.assign(c, ot.max(c, 0.0) * ot.scalar(dt))], [[-1, -2]])
+  lines, routines = [], {}
+
+  class Cache:
+    def addRoutine(self, name, writer):
+      routines[name] = writer
+
+  fe.generate(lines.append, Cache())
+  assert len(routines) == 2 and len(lines) == 2
+  assert lines[0].startswith('launcher_kernel_') and 'const_cast<const double*>(A)' in lines[0]
+  assert lines[1].endswith('(C, extraOffset_C, dt, numElements, flags, streamPtr);')
+  pointwise = list(routines.values())[1]._generator
+  assert pointwise.get_emitted().schedule['kind'] == 'pointwise'
+  assert c.indices == [-1, -2] and c.tensor.bbox.sizes() == [8, 3]

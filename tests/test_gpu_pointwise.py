@@ -1,0 +1,126 @@
+"""Parity of the pointwise kernels (ElementwiseDescr) with the pointwise oracle, through the C ABI
+on the GPU; full-size checks through closed forms and invariants.
+
+Tolerance (normwise per element, ``harness.normwise_error``): 1e-12 for FP64 kernels, 1e-5 for
+FP32 kernels; integer outputs must match exactly.
+"""
+import numpy as np
+import pytest
+
+from tensorforge_amd import catalog
+
+from harness import TOL, gpu_run, key_of, make_inputs, normwise_error, oracle_outputs, outputs_of
+
+pytestmark = pytest.mark.gpu
+
+POINTWISE = ['pw_update_f64', 'pw_update_f32', 'pw_plastic_f64', 't_pw_types', 't_pw_math_f32']
+
+
+def _check(entry, kernel, device, n, seed=0, offsets=None, flags=None):
+  print(f'pointwise {entry.name} N={n} seed={seed} offsets={offsets} flags={flags is not None}',
+        flush=True)
+  host = make_inputs(entry, n, seed=seed, offsets=offsets)
+  ref = oracle_outputs(entry, host, n, offsets=offsets, flags=flags)
+  got = gpu_run(kernel, entry, host, n, device, offsets=offsets, flags=flags)
+  tol = TOL[entry.fp_type]
+  for t in entry.generator._global_tensors:
+    key = key_of(t)
+    if key not in outputs_of(entry) or key not in got:
+      continue
+    off = (offsets or {}).get(key, 0)
+    integral = np.issubdtype(np.asarray(host[key] if not isinstance(host[key], list)
+                                        else host[key][0]).dtype, np.integer)
+    if isinstance(ref[key], list):
+      g = np.stack([x[off:] for x in got[key]]).astype(np.float64)
+      r = np.stack([x[off:] for x in ref[key]])
+      err = normwise_error(list(g), list(r), t.get_real_volume())
+    else:
+      np.testing.assert_array_equal(got[key][:off], host[key][:off])
+      if integral:
+        np.testing.assert_array_equal(got[key].astype(np.float64), ref[key])
+        err = 0.0
+      else:
+        err = normwise_error(got[key], ref[key], t.get_real_volume(), off)
+    print(f'{entry.name} {key} N={n} normwise rel err {err:.3e} (tol {tol})', flush=True)
+    assert err <= tol, f'{entry.name}: {key} normwise error {err:.3e} > {tol}'
+
+
+@pytest.mark.parametrize('name', POINTWISE)
+def test_pointwise_matches_oracle(name, device):
+  entry, kernel = catalog.load(name)
+  for n in (1, 3, 37, 301):
+    _check(entry, kernel, device, n, seed=n)
+
+
+def test_pointwise_flags_and_offsets(device):
+  entry, kernel = catalog.load('pw_plastic_f64')
+  n = 45
+  flags = (np.arange(n) % 3 != 1).astype(np.uint32)
+  _check(entry, kernel, device, n, seed=5, offsets={'Q': 11}, flags=flags)
+  entry, kernel = catalog.load('t_pw_types')
+  _check(entry, kernel, device, 50, seed=6, offsets={'A': 3, 'K': 5, 'D': 2, 'M': 9},
+         flags=(np.arange(50) % 4 != 0).astype(np.uint32))
+
+
+def test_pointwise_zero_elements(device):
+  entry, kernel = catalog.load('pw_update_f64')
+  host = make_inputs(entry, 4, seed=1)
+  got = gpu_run(kernel, entry, host, 0, device)
+  np.testing.assert_array_equal(got['Q'], host['Q'])
+
+
+def test_update_full_size_closed_form(device):
+  """262144 elements: Q + dt * I entry by entry against torch's FP64 result. The kernel may
+  fuse the multiply-add (one rounding) where torch rounds twice, so the bound is 2 units of
+  roundoff of the terms' magnitude, ``|Q| + |dt * I|``, not of the (possibly cancelled) result."""
+  import torch
+  from tensorforge_amd.device import launch
+  entry, kernel = catalog.load('pw_update_f64')
+  n = 262144
+  g = torch.Generator(device=device).manual_seed(11)
+  Q = torch.rand(n * 504, dtype=torch.float64, device=device, generator=g) - 0.5
+  I = torch.rand(n * 504, dtype=torch.float64, device=device, generator=g) - 0.5
+  want = Q + 0.37 * I
+  scale = Q.abs() + (0.37 * I).abs()
+  print(f'pointwise update full size N={n}', flush=True)
+  launch(kernel, {'Q': Q, 'I': I, 'dt': 0.37}, n)
+  torch.cuda.synchronize()
+  rel = ((Q - want).abs() / scale.clamp_min(1e-300)).max().item()
+  print(f'max diff vs torch, relative to |Q| + |dt I|: {rel:.3e}', flush=True)
+  assert rel <= 2.3e-16
+
+
+def test_plastic_full_size_invariants(device):
+  """262144 elements of 56 nodes: the mean stress is kept, yielded nodes end on the yield
+  surface, elastic nodes and velocities are untouched."""
+  import torch
+  from tensorforge_amd.device import launch
+  entry, kernel = catalog.load('pw_plastic_f64')
+  n = 262144
+  g = torch.Generator(device=device).manual_seed(12)
+  Q = torch.rand(n * 504, dtype=torch.float64, device=device, generator=g) * 2 - 1
+  before = Q.clone().view(n, 9, 56)
+  print(f'pointwise plastic full size N={n}', flush=True)
+  launch(kernel, {'Q': Q, 'cohesion': 1.2, 'tanphi': 0.2}, n)
+  torch.cuda.synchronize()
+  after = Q.view(n, 9, 56)
+
+  def inv(q):
+    s = q[:, :6, :]
+    m = s[:, :3].mean(dim=1)
+    dev = s[:, :3] - m[:, None]
+    return m, torch.sqrt(0.5 * (dev ** 2).sum(1) + (s[:, 3:] ** 2).sum(1))
+  m0, tau0 = inv(before)
+  m1, tau1 = inv(after)
+  yld = 1.2 - 0.2 * m0
+  yielded = tau0 > yld
+  frac = yielded.double().mean().item()
+  err_m = (m1 - m0).abs().max().item()
+  err_tau = ((tau1 - yld).abs() / yld)[yielded].max().item()
+  print(f'yielded {frac:.3f} mean-stress drift {err_m:.2e} surface rel err {err_tau:.2e}',
+        flush=True)
+  assert 0.2 < frac < 0.8
+  assert err_m <= 4e-15 and err_tau <= 1e-13
+  elastic = (~yielded)[:, None, :].expand(n, 9, 56)
+  assert torch.equal(after[elastic], before[elastic])
+  assert torch.equal(after[:, 6:], before[:, 6:])
