@@ -5,9 +5,9 @@ _HERE = os.path.dirname(os.path.realpath(__file__))
 
 
 def get_cmake_path():
-  """Directory of CMake package files. The package ships a ctypes/C-ABI library instead of a
-  CMake package; the path names where ``tensorforge_aux.h`` and the C headers live."""
-  return os.path.join(os.path.dirname(_HERE), 'include')
+  """Directory of the CMake package files (``TensorForgeConfig.cmake``): pass it as
+  ``TensorForge_DIR`` or on ``CMAKE_PREFIX_PATH`` for ``find_package(TensorForge)``."""
+  return os.path.join(_HERE, 'share', 'cmake')
 
 
 def print_cmake_path():

@@ -1,0 +1,60 @@
+// C++ consumer of a generated routine file: C_e = A * B_e (a contraction kernel), then
+// C_e = max(C_e, 0) * dt (a pointwise kernel), launched through the call sites the yateto
+// frontend emitted (tf_execute in kernels.h). Checked against a plain CPU loop.
+// Exit status: 0 agreement, 1 disagreement, 2 a device or launch error.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "kernels.h"
+#include "tensorforge_aux.h"
+
+int main() {
+  const size_t n = 1000;
+  const int b = 56, q = 9;
+  const double dt = 0.5;
+  std::vector<double> A(b * b), B(n * b * q), C(n * b * q, 0.0);
+  std::mt19937_64 rng(7);
+  std::uniform_real_distribution<double> u(-1.0, 1.0);
+  for (auto& x : A) x = u(rng);
+  for (auto& x : B) x = u(rng);
+
+  double *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  if (hipMalloc(&dA, A.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc(&dB, B.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc(&dC, C.size() * sizeof(double)) != hipSuccess) {
+    std::fprintf(stderr, "hipMalloc failed\n");
+    return 2;
+  }
+  hipMemcpy(dA, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice);
+  hipMemcpy(dB, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice);
+  hipMemcpy(dC, C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice);
+
+  tf_execute(dA, dB, dC, dt, 0u, 0u, n, nullptr, nullptr);
+  if (tensorforge::synchDevice() != 0 || CHECK_ERR != 0) return 2;
+  hipMemcpy(C.data(), dC, C.size() * sizeof(double), hipMemcpyDeviceToHost);
+  hipFree(dA);
+  hipFree(dB);
+  hipFree(dC);
+
+  double worst = 0.0;
+  for (size_t e = 0; e < n; ++e) {
+    double diff = 0.0, norm = 0.0;
+    for (int j = 0; j < q; ++j) {
+      for (int i = 0; i < b; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < b; ++k) s += A[i + k * b] * B[e * b * q + k + j * b];
+        const double want = std::fmax(s, 0.0) * dt;
+        const double got = C[e * b * q + i + j * b];
+        diff = std::fmax(diff, std::fabs(got - want));
+        norm = std::fmax(norm, std::fabs(want));
+      }
+    }
+    worst = std::fmax(worst, norm > 0.0 ? diff / norm : diff);
+  }
+  std::printf("cmake consumer: %zu elements, max normwise error %.3e\n", n, worst);
+  return worst <= 1e-12 ? 0 : 1;
+}
