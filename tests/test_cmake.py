@@ -56,3 +56,28 @@ def test_other_backends_are_refused(tmp_path):
                       f'-DTensorForge_DIR={tensorforge.get_cmake_path()}',
                       '-DDEVICE_BACKEND=cuda'], capture_output=True, text=True)
   assert r.returncode != 0 and 'Allowed: hip' in (r.stdout + r.stderr)
+
+
+@pytest.mark.skipif(shutil.which('cmake') is None, reason='cmake is not installed')
+def test_reference_integration_project(tmp_path):
+  """The reference's own CMake integration check (tests/cmake_integration/CMakeLists.txt:1-11):
+  CMAKE_PREFIX_PATH from print_cmake_path(), find_package(Tensorforge REQUIRED), the include
+  directories, a C++ file including tensorforge_aux.h."""
+  (tmp_path / 'CMakeLists.txt').write_text(
+    'cmake_minimum_required(VERSION 3.5)\nproject(cmake_integration)\n'
+    'execute_process(COMMAND python3 -c "import tensorforge; tensorforge.print_cmake_path()"\n'
+    '                OUTPUT_VARIABLE TENSORFORGE_PATH)\n'
+    'set(CMAKE_PREFIX_PATH "${TENSORFORGE_PATH}" ${CMAKE_MODULE_PATH})\n'
+    'find_package(Tensorforge REQUIRED)\n'
+    'add_executable(${CMAKE_PROJECT_NAME} ./main.cpp)\n'
+    'target_include_directories(${CMAKE_PROJECT_NAME} PRIVATE ${TensorForge_INCLUDE_DIRS})\n')
+  (tmp_path / 'main.cpp').write_text(
+    '#include <tensorforge_aux.h>\n#include <iostream>\n'
+    'int main() { std::cout << "hello tensorforge" << std::endl; return 0; }\n')
+  env = dict(os.environ, PYTHONPATH=ROOT)
+  for cmd in (['cmake', '-S', str(tmp_path), '-B', str(tmp_path / 'b')],
+              ['cmake', '--build', str(tmp_path / 'b')]):
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env)
+    assert r.returncode == 0, (r.stdout + r.stderr)[-3000:]
+  r = subprocess.run([str(tmp_path / 'b' / 'cmake_integration')], capture_output=True, text=True)
+  assert r.returncode == 0 and 'hello tensorforge' in r.stdout
