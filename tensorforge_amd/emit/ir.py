@@ -104,7 +104,13 @@ class Product:
     return [hi - lo for lo, hi in self.n_ranges]
 
   def is_empty(self):
+    """No output entry or no contraction term: nothing to multiply."""
     return any(hi <= lo for lo, hi in self.n_ranges + self.k_ranges)
+
+  def out_empty(self):
+    """No output entry: the product writes nothing. (An empty contraction with outputs still
+    writes them: ``alpha * 0 + beta * dest``, zero when not accumulating.)"""
+    return any(hi <= lo for lo, hi in self.n_ranges)
 
   def dest_offset(self, n):
     return sum((i - o) * s for i, o, s in zip(n, self.dest_origin, self.dest_strides))
@@ -261,6 +267,8 @@ def _lower_one(descr, by_id, tmp_origin, tmp_shape) -> Product:
   for c, r in enumerate(k_ranges):
     if r is None:
       raise GenerationError(f'contracted index {-c - 1} is missing from every operand')
+  # disjoint boxes give an empty range: keep its lower end, size zero
+  k_ranges = [(lo, max(lo, hi)) for lo, hi in k_ranges]
 
   dest = descr.dest.tensor
   if len(dest.shape) != n_rank and not dest.is_tmp:
@@ -271,7 +279,8 @@ def _lower_one(descr, by_id, tmp_origin, tmp_shape) -> Product:
       # accumulate into (or overwrite) an existing temporary: keep its coordinates
       origin = tmp_origin[id(dest)]
       shape = tmp_shape[id(dest)]
-      n_ranges = [(max(lo, o), min(hi, o + s)) for (lo, hi), o, s in zip(n_ranges, origin, shape)]
+      n_ranges = [(max(lo, o), max(max(lo, o), min(hi, o + s)))
+                  for (lo, hi), o, s in zip(n_ranges, origin, shape)]
     else:
       if descr.add:
         raise GenerationError(f'cannot accumulate into temporary {dest.name} before it is produced')
@@ -291,6 +300,7 @@ def _lower_one(descr, by_id, tmp_origin, tmp_shape) -> Product:
   view = descr.dest.bbox
   n_ranges = [(max(lo, vl, tl), min(hi, vu, tu)) for (lo, hi), vl, vu, tl, tu in
               zip(n_ranges, view.lower(), view.upper(), dest.bbox.lower(), dest.bbox.upper())]
+  n_ranges = [(lo, max(lo, hi)) for lo, hi in n_ranges]
   return Product(dest=dest, dest_src=SRC_GLOBAL, dest_arg=arg, dest_origin=tuple(dest.bbox.lower()),
                  dest_strides=colmajor_strides(dest.shape), n_ranges=n_ranges, k_ranges=k_ranges,
                  factors=factors, scalars=scalars, add=descr.add, beta=descr.beta, descr=descr)

@@ -143,7 +143,7 @@ def _emit_product(w, program, p, pi, plan, lanes, fp, tmp_in_regs):
   real = _real(fp)
   w()
   w(f'// product {pi}: {p.descr}')
-  if p.is_empty():
+  if p.out_empty():
     if p.dest_src == ir.SRC_TMP and id(p.dest) not in tmp_in_regs:
       raise GenerationError(f'temporary {p.dest.name} has an empty range')
     return
@@ -167,6 +167,8 @@ def _emit_product(w, program, p, pi, plan, lanes, fp, tmp_in_regs):
           _, off, window, ldsizes = info
           sizes = [hi - lo for lo, hi in window]
           count = ir._prod(sizes)
+          if count == 0:       # an empty contraction range stages nothing
+            continue
           with w.block(f'for (int i = lane; i < {count}; i += {lanes})'):
             w('int r = i;')
             gidx, lidx, lstride = [], [], 1
@@ -381,7 +383,9 @@ def _term_lines(p, plan, c, lanes, fp, acc, chunks, tmp_shapes, tmp_in_regs):
       a = f'{acc}[{c + chunks * r_idx}]'
       out.append(f'{a} = {a} + {prod};')
 
-  if _runtime_k_ok(p, plan):
+  if any(hi <= lo for lo, hi in p.k_ranges):
+    pass                 # empty contraction: the accumulator stays zero
+  elif _runtime_k_ok(p, plan):
     total = ir._prod([hi - lo for lo, hi in p.k_ranges])
     kvars = [f'k{i}' for i in range(len(p.k_ranges))]
     lines.append('#pragma unroll 2')

@@ -30,7 +30,7 @@ def _factor_offsets(p, f):
 def element_traffic(program: ir.ChainProgram) -> Traffic:
   read, written = {}, {}
   for p in program.products:
-    if p.is_empty():
+    if p.out_empty():
       continue
     for f in p.factors:
       if f.src != ir.SRC_GLOBAL:
