@@ -35,7 +35,7 @@ def parse():
   ap.add_argument('--cpu-seconds', type=float, default=25.0)
   ap.add_argument('--only', default='',
                   help='comma list of legs (tuned,plain,f32,copy,mixed,precision,flux,tuner,'
-                       'pointwise)')
+                       'pointwise,gemm)')
   return ap.parse_args()
 
 
@@ -272,6 +272,69 @@ def flux_leg(device, steps, warmup, n=262144):
   return res
 
 
+def gemm_leg(device, steps, warmup, cpu_seconds, with_cpu, n=65536):
+  """Single batched GEMM C_e += A * B_e (A 56x56 shared, B_e and C_e 56x9, seed 0, FP64,
+  65,536 elements): tuned and plain launchers with the HBM roofline (12,096 B per element), the
+  tuned result against the closed form on every element, and the OpenMP restatement of the
+  reference's generated kernel (oracle/omp_chain.cpp, team and flat forms) on the host."""
+  import numpy as np
+  import torch
+  from tensorforge_amd import catalog
+  from tensorforge_amd import workloads as W
+  from tensorforge_amd.common.basic_types import FloatingPointType
+  entry, tuned = catalog.load('gemm_f64_tuned')
+  _, plain = catalog.load('gemm_f64_plain')
+  w = W.gemm(FloatingPointType.DOUBLE)
+  host = W.gemm_inputs(w, n, seed=0)
+  b, q = w.meta['b'], w.meta['q']
+  want = host['C'].reshape(n, q, b) + np.einsum('ik,ekj->eji', host['A'].reshape(b, b).T,
+                                                host['B'].reshape(n, q, b).transpose(0, 2, 1))
+  res = {'elements': n, 'unit': 'elements/s'}
+  em = tuned.generator.get_emitted()
+  bpl = em.bytes_per_element * n + em.shared_bytes
+  for label, k in (('tuned', tuned), ('plain', plain)):
+    bufs = {key: torch.from_numpy(v.copy()).to(device) for key, v in host.items()}
+    from tensorforge_amd.device import launch
+    launch(k, bufs, n)
+    torch.cuda.synchronize()
+    got = bufs['C'].cpu().numpy().reshape(n, q, b)
+    err = float((np.abs(got - want).reshape(n, -1).max(axis=1) /
+                 np.abs(want).reshape(n, -1).max(axis=1)).max())
+    t = time_kernel(k, bufs, n, steps if label == 'tuned' else max(3, steps // 5), warmup)
+    res[label] = {'value': n / t, 'ms_per_step': t * 1e3, 'hbm_GBps': bpl / t / 1e9,
+                  'frac_of_peak': bpl / t / 1e9 / HBM_PEAK_GBPS,
+                  'max_normwise_rel_err_vs_closed_form': err}
+    del bufs
+  res['tuned_over_plain'] = res['plain']['ms_per_step'] / res['tuned']['ms_per_step']
+  res['schedule'] = em.schedule
+  if with_cpu:
+    lib = ctypes.CDLL(os.path.join(ROOT, 'oracle', 'lib', 'libomp_chain.so'))
+    DP = ctypes.POINTER(ctypes.c_double)
+    cpu = {'cores': lib.omp_max_threads(), 'kind': 'port'}
+    for form in ('team', 'flat'):
+      m = 256
+      while True:
+        C = host['C'][:m * b * q].copy()
+        t0 = time.perf_counter()
+        if form == 'team':
+          lib.omp_gemm_team(host['A'].ctypes.data_as(DP), host['B'].ctypes.data_as(DP),
+                            C.ctypes.data_as(DP), b, q, ctypes.c_long(m), 32, 8)
+        else:
+          lib.omp_gemm_flat(host['A'].ctypes.data_as(DP), host['B'].ctypes.data_as(DP),
+                            C.ctypes.data_as(DP), b, q, ctypes.c_long(m))
+        t = time.perf_counter() - t0
+        if t > cpu_seconds / 4 or m == n:
+          break
+        m = min(n, m * 4)
+      ref = C.reshape(m, q, b)
+      cpu[form] = {'value': m / t, 'elements': m, 'seconds': t,
+                   'max_normwise_rel_err_vs_closed_form': float(
+                     (np.abs(ref - want[:m]).reshape(m, -1).max(axis=1) /
+                      np.abs(want[:m]).reshape(m, -1).max(axis=1)).max())}
+    res['cpu_baseline'] = cpu
+  return res
+
+
 def pointwise_leg(device, steps, warmup, n=262144):
   """Pointwise kernels (ElementwiseDescr) at the headline element count: the update
   Q += dt * I (FP64 and FP32, 2 reads + 1 write per entry) and the plastic return map on the six
@@ -406,7 +469,8 @@ def main():
   from tensorforge_amd import catalog
   device = torch.device('cuda:0')
   legs = set(args.only.split(',')) if args.only else {'tuned', 'plain', 'f32', 'copy', 'mixed',
-                                                      'precision', 'flux', 'tuner', 'pointwise'}
+                                                      'precision', 'flux', 'tuner', 'pointwise',
+                                                      'gemm'}
   n = args.elements
   out = {}
 
@@ -473,6 +537,10 @@ def main():
 
   if 'pointwise' in legs:
     out['pointwise'] = pointwise_leg(device, args.steps, args.warmup)
+
+  if 'gemm' in legs:
+    out['single_gemm'] = gemm_leg(device, args.steps, args.warmup, min(args.cpu_seconds, 8.0),
+                                  not args.no_cpu)
 
   if 'tuner' in legs:
     out['cost_model'] = tuner_leg()

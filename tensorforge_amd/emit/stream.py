@@ -329,6 +329,7 @@ class _Stream:
   # ---------------------------------------------------------------- emission
   def emit(self) -> Emitted:
     pl = self.pl
+    self._g0_declared = False
     self.order_entries()
     self.loop = self._loop_mode()
     if self.loop and self.KB != self.sched.stream_loop_k_block:
@@ -989,13 +990,25 @@ class _Stream:
       w('__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");')
       w('__builtin_amdgcn_wave_barrier();')
       w('__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");')
-    # right-operand rows of block bi, as plain values for compute_block
+    # right-operand rows of block bi, as plain values for compute_block. Block 0 is written
+    # before the element-group loop and again at the end of every group for the next one, and
+    # its compute runs at the top of the loop body: its values live in variables carried across
+    # iterations (declared by the first write, assigned by the later ones).
     vals = {}
     for ti in self.gterms:
       for j, k in enumerate(range(b0, b1)):
-        v = self.fresh('gr')
-        w(f'const {self.real} {v} = pg{reg}_{ti}[{j}];')
+        if bi == 0:
+          v = f'g0r_{ti}_{j}'
+          if self._g0_declared:
+            w(f'{v} = pg{reg}_{ti}[{j}];')
+          else:
+            w(f'{self.real} {v} = pg{reg}_{ti}[{j}];')
+        else:
+          v = self.fresh('gr')
+          w(f'const {self.real} {v} = pg{reg}_{ti}[{j}];')
         vals[(ti, k)] = v
+    if bi == 0:
+      self._g0_declared = True
     self.pipe_global = [vals]
 
   def pipeline_realign(self):

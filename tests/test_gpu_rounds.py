@@ -1,0 +1,41 @@
+"""Persistent kernels over many element groups per workgroup.
+
+The streaming, column-per-lane and matrix-core kernels size their grid to the resident
+workgroups and loop over element groups, carrying prefetched rows from one group to the next.
+The parity tests' batches fit in one round; here every tuned launcher runs 60,001 elements
+(at least two rounds on MI355X, a partial last group) and is compared on every element with the
+plain launcher of the same chain, which has no cross-group state (itself checked against the
+oracle). Normwise per element: 1e-12 FP64, 1e-5 FP32 and FP32 MFMA, 1e-4 split BF16x3.
+"""
+import pytest
+
+from tensorforge_amd import catalog
+
+from harness import gpu_run, key_of, make_inputs, normwise_error, outputs_of
+
+pytestmark = pytest.mark.gpu
+
+N = 60001
+PAIRS = [(t, t.rsplit('_', 1)[0] + '_plain', 1e-5 if '_f32_' in t else 1e-12)
+         for t in catalog.names() if t.endswith('_tuned') and not t.startswith('t_')]
+PAIRS += [('ader7_f64_mfma_fp32', 'ader7_f64_plain', 1e-5),
+          ('ader7_f64_mfma_bf16x3', 'ader7_f64_plain', 1e-4)]
+PAIRS += [(t, t[:-len('_auto')] + '_plain', 1e-5 if 'f32' in t else 1e-12)
+          for t in catalog.names() if t.startswith('t_') and t.endswith('_auto')]
+
+
+@pytest.mark.parametrize('tuned,plain,tol', PAIRS)
+def test_many_rounds_match_plain(tuned, plain, tol, device):
+  et, kt = catalog.load(tuned)
+  ep, kp = catalog.load(plain)
+  print(f'{tuned} ({et.generator.get_emitted().emitter}) vs {plain}, N={N}', flush=True)
+  host = make_inputs(ep, N, seed=17)
+  a = gpu_run(kp, ep, host, N, device)
+  b = gpu_run(kt, et, host, N, device)
+  for t in ep.generator._global_tensors:
+    key = key_of(t)
+    if key not in outputs_of(ep):
+      continue
+    err = normwise_error(b[key], a[key], t.get_real_volume())
+    print(f'{tuned} {key}: normwise rel diff {err:.3e} (tol {tol})', flush=True)
+    assert err <= tol
