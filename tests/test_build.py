@@ -15,6 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared(header):
   text = open(os.path.join(ROOT, 'include', header)).read()
   text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+  text = '\n'.join(l for l in text.splitlines() if not l.lstrip().startswith('#'))
   return re.findall(r'\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;', text)
 
 
@@ -73,3 +74,23 @@ def test_loader_binds_without_gpu_calls():
   assert [t for t, _ in k.signature][-3:] == ['size_t', 'unsigned*', 'void*']
   with pytest.raises(TypeError):
     k(1, 2)
+
+
+def test_shared_launcher_names_resolve_per_object():
+  """Chains that differ only in floating type or sparsity share a launcher name; with its
+  TF_CATALOG_<ENTRY> macro defined the header declares exactly that object's prototype."""
+  from tensorforge_amd import catalog_header
+  header = os.path.join(ROOT, 'include', 'tensorforge_catalog.h')
+  for name in ('ader_f64_tuned', 'ader_f32_tuned', 'gemm_f64_tuned', 'gemm_f32_plain',
+               'pw_update_f64', 'pw_update_f32'):
+    g = catalog.get(name).generator
+    g.register()
+    proto = g._generate_launcher_proto(with_defaults=False).replace('unsigned* flags ',
+                                                                    'unsigned* flags')
+    out = subprocess.run(['gcc', '-E', '-P', '-x', 'c', f'-D{catalog_header.macro(name)}', header],
+                         stdout=subprocess.PIPE, text=True, check=True).stdout
+    decls = [l for l in out.splitlines() if f'launcher_{g.get_base_name()}(' in l]
+    assert decls == [proto + ';'], (name, decls)
+  r = subprocess.run(['gcc', '-fsyntax-only', '-x', 'c', '-DTF_CATALOG_ADER_F64_TUNED', header],
+                     capture_output=True, text=True)
+  assert r.returncode == 0, r.stderr
