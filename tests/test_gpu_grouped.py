@@ -128,3 +128,34 @@ def test_graph_replay_of_ping_pong_steps(mixed, device):
       err = normwise_error(got, w, vol)
       assert err <= 1e-14, (replay, e.meta['b'], err)
   graph.close()
+
+
+def test_many_rounds_grouped_and_members_match_plain(mixed, device):
+  """Counts that give every member several element groups per workgroup: the grouped launch and
+  each member's tuned launcher (autotuned schedules) against the member's plain launcher, which
+  has no cross-group state, on every element."""
+  import torch
+  from tensorforge_amd.device import launch
+  group, tuned, plain, gk = mixed
+  counts = [30011, 60013, 40007, 35003, 30001, 30007]
+  print(f'grouped and tuned members vs plain, counts {counts}', flush=True)
+  hosts = _member_inputs(group, counts, seed=31)
+  ref, solo = [], []
+  for kp, kt, host, n in zip(plain, tuned, hosts, counts):
+    bufs = _device_bufs(host, device)
+    launch(kp, bufs, n)
+    ref.append(bufs['Qp'].cpu().numpy())
+    bufs = _device_bufs(host, device)
+    launch(kt, bufs, n)
+    solo.append(bufs['Qp'].cpu().numpy())
+    torch.cuda.synchronize()
+  together = [_device_bufs(h, device) for h in hosts]
+  gk(*_grouped_args(group, gk, together, counts))
+  torch.cuda.synchronize()
+  for e, r, s, t, n in zip(group.members, ref, solo, together, counts):
+    vol = e.tensors['Qp'].get_real_volume()
+    err_solo = normwise_error(s, r, vol)
+    err_group = normwise_error(t['Qp'].cpu().numpy(), r, vol)
+    print(f'b={e.meta["b"]} n={n}: tuned member vs plain {err_solo:.2e}, grouped vs plain '
+          f'{err_group:.2e}', flush=True)
+    assert err_solo <= 1e-12 and err_group <= 1e-12
