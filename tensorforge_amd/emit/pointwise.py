@@ -447,7 +447,7 @@ def plan(points):
 
 
 def emit(prog: PointwiseProgram, context, base_name: str) -> Emitted:
-  kname = f'{base_name}'
+  kname = base_name
   P = prog.points
   epb, items = plan(P)
   PB = epb * P

@@ -35,15 +35,6 @@ from ..common.matrix.tensor import SubTensor, Tensor
 from ..common.operation import Operation
 
 
-class VarAlloc:
-  def __init__(self):
-    self.counter = -1
-
-  def alloc(self):
-    self.counter += 1
-    return f'v{self.counter}'
-
-
 class Node:
   """An expression. Operators build trees; ``==`` builds a comparison, so nodes hash by
   identity."""

@@ -11,7 +11,6 @@ from tensorforge_amd import catalog
 from tensorforge_amd.common.basic_types import Addressing, DataFlowDirection, FloatingPointType
 from tensorforge_amd.common.context import Context
 from tensorforge_amd.common.exceptions import GenerationError
-from tensorforge_amd.common.matrix.boundingbox import BoundingBox
 from tensorforge_amd.common.matrix.spp import ListSPP
 from tensorforge_amd.common.matrix.tensor import SubTensor, Tensor
 from tensorforge_amd.common.operation import Operation
