@@ -428,3 +428,28 @@ def _pw_math_entry():
               + ot.pow(u, 1.7) + ot.sqrt(u) / (u + 1.0) + ot.asinh(x) + ot.cosh(x) * 0.1),
   ]
   return Entry('', [ElementwiseDescr(stmts)], F32, {'X': X, 'Y': Y})
+
+
+# ---- schedule options of the streaming emitter (each built and tested, none the default) ----
+STREAM_OPTIONS = {
+  'k4': dict(stream_k_block=4),
+  'la0': dict(stream_lds_lookahead=0),
+  'pf2': dict(stream_prefetch=2),
+  'w4': dict(stream_waves=4),
+  'pair': dict(stream_pair_reads=True),
+  'rows': dict(stream_row_loads=True),
+  'dma': dict(stream_dma=True),
+}
+
+
+def _option_entry(w, kw):
+  from .emit.tuned import Schedule
+  return _wl_entry(w, 'tuned', Schedule(**kw))
+
+
+for _tag, _kw in STREAM_OPTIONS.items():
+  register(f'opt_ader_f64_{_tag}')(lambda kw=_kw: _option_entry(W.ader_volume(F64), kw))
+  if _tag not in ('dma', 'rows'):      # those two apply to chains that stream through the ring
+    register(f'opt_gemm_f64_{_tag}')(lambda kw=_kw: _option_entry(W.gemm(F64), kw))
+register('opt_ader_dense_f64_unrolled')(
+  lambda: _option_entry(W.ader_volume(F64, sparse_k=False), dict(stream_loop='never')))

@@ -22,6 +22,9 @@ PAIRS += [('ader7_f64_mfma_fp32', 'ader7_f64_plain', 1e-5),
           ('ader7_f64_mfma_bf16x3', 'ader7_f64_plain', 1e-4)]
 PAIRS += [(t, t[:-len('_auto')] + '_plain', 1e-5 if 'f32' in t else 1e-12)
           for t in catalog.names() if t.startswith('t_') and t.endswith('_auto')]
+# the streaming emitter's schedule options (catalog opt_<chain>_<option>)
+PAIRS += [(t, t[len('opt_'):t.rindex('_')] + '_plain', 1e-12)
+          for t in catalog.names() if t.startswith('opt_')]
 
 
 @pytest.mark.parametrize('tuned,plain,tol', PAIRS)
@@ -43,7 +46,9 @@ def test_many_rounds_match_plain(tuned, plain, tol, device):
 
 @pytest.mark.parametrize('tuned,plain', [('ader_f64_tuned', 'ader_f64_plain'),
                                          ('gemm_f64_tuned', 'gemm_f64_plain'),
-                                         ('flux_neigh_mod_f64_tuned', 'flux_neigh_mod_f64_plain')])
+                                         ('flux_neigh_mod_f64_tuned', 'flux_neigh_mod_f64_plain'),
+                                         # odd offsets: the direct-to-LDS loads' fallback kernel
+                                         ('opt_ader_f64_dma', 'ader_f64_plain')])
 def test_many_rounds_with_flags_and_offsets(tuned, plain, device):
   """Skipped elements and extra offsets across rounds: untouched entries stay untouched."""
   import numpy as np
