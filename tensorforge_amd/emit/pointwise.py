@@ -18,6 +18,10 @@ for HBM streaming instead of per element:
   for all ``ITEMS`` points first (all loads in flight at once), the statements then run on
   registers (a tensor entry written and read again is forwarded, not re-read), and the written
   entries are stored last;
+* per-element entries whose first index runs along the unit-stride dimension are touched once
+  in whole cache lines, so they are loaded and stored non-temporally (measured on the update
+  ``Q += dt * I``: 6.0 TB/s against 5.4 TB/s with ordinary loads); shared operands and other
+  access patterns keep ordinary, cached accesses;
 * ``P > 256 * ITEMS`` loops over ``256 * ITEMS`` point chunks of one element.
 
 The three-phase form assumes what the reference's per-lane form already assumes: points are
@@ -412,6 +416,14 @@ def _address(cell: Cell, e: str, n: List[str]) -> str:
   return f'{name}[{inner}]'
 
 
+def _streamed(cell: Cell) -> bool:
+  """A per-element entry that consecutive lanes read at consecutive addresses (the first index
+  runs along the tensor's first, unit-stride dimension): each wavefront access covers whole cache
+  lines, read once, so it bypasses the caches. Other patterns rely on the caches to reuse the
+  rest of a line and keep ordinary accesses."""
+  return cell.arg.per_element and len(cell.indices) > 0 and cell.indices[0] == -1
+
+
 def _tables(prog, w):
   """Constant-valued tensors read at run-time indices live in device tables."""
   done = set()
@@ -512,7 +524,12 @@ def _chunk(prog, w, items, P, PB, epb):
       if c.written:
         w(f'bool w{c.index}{s} = false;')
       if c.load:
-        w(f'if (ok{s}) c{c.index}{s} = {_address(c, "e" + s, n)};')
+        addr = _address(c, "e" + s, n)
+        if _streamed(c):
+          # entries of one element are read once: do not keep them in the caches
+          w(f'if (ok{s}) c{c.index}{s} = __builtin_nontemporal_load(&{addr});')
+        else:
+          w(f'if (ok{s}) c{c.index}{s} = {addr};')
   # phase 2: the statements on registers
   for k in range(items):
     s = f'_{k}'
@@ -527,4 +544,8 @@ def _chunk(prog, w, items, P, PB, epb):
     n = [f'n{d}{s}' for d in range(rank)]
     for c in prog.cells:
       if c.written:
-        w(f'if (ok{s} && w{c.index}{s}) {_address(c, "e" + s, n)} = c{c.index}{s};')
+        addr = _address(c, "e" + s, n)
+        if _streamed(c):
+          w(f'if (ok{s} && w{c.index}{s}) __builtin_nontemporal_store(c{c.index}{s}, &{addr});')
+        else:
+          w(f'if (ok{s} && w{c.index}{s}) {addr} = c{c.index}{s};')
