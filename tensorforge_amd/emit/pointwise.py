@@ -446,13 +446,16 @@ def _tables(prog, w):
     w(f'static __device__ const {c.dtype} {c.arg.name}_values[{vol}] = {{{body}}};')
 
 
-def plan(points):
-  """(elements per block, items per lane) for ``points`` points per element."""
-  chunk = THREADS * ITEMS
+def plan(points, entry_bytes=8):
+  """(elements per block, items per lane) for ``points`` points per element; entries narrower
+  than 8 bytes get proportionally more points per lane, so a lane keeps as many bytes in
+  flight."""
+  items_max = ITEMS * max(1, 8 // max(1, entry_bytes))
+  chunk = THREADS * items_max
   if points == 0:
     return 64, 1
   if points >= chunk:
-    return 1, ITEMS
+    return 1, items_max
   epb = chunk // points
   items = -(-(epb * points) // THREADS)
   return epb, items
@@ -461,7 +464,8 @@ def plan(points):
 def emit(prog: PointwiseProgram, context, base_name: str) -> Emitted:
   kname = base_name
   P = prog.points
-  epb, items = plan(P)
+  widths = [c.dtype.size() for c in prog.cells if c.arg.per_element]
+  epb, items = plan(P, max(widths) if widths else 8)
   PB = epb * P
   w = Writer()
   _tables(prog, w)
