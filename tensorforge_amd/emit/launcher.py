@@ -152,12 +152,20 @@ def _launch_block(w, generator, emitted):
 
 
 def translation_unit(generators) -> str:
+  """One HIP source holding every generator's kernel and launcher. A name appears once; two
+  different kernels under one name (descriptions that differ only where the name's hash does
+  not look: floating type, sparsity pattern, constant values) cannot share an object."""
+  from ..common.exceptions import GenerationError
   parts = [HEADER]
-  seen = set()
+  seen = {}
   for g in generators:
-    if g.get_base_name() in seen:
+    name = g.get_base_name()
+    if name in seen:
+      if seen[name] != (g.get_kernel(), g.get_launcher()):
+        raise GenerationError(f'two different kernels are named {name}: build them into '
+                              'separate objects')
       continue
-    seen.add(g.get_base_name())
+    seen[name] = (g.get_kernel(), g.get_launcher())
     parts.append(g.get_kernel())
     parts.append(g.get_launcher())
   return '\n'.join(parts)

@@ -10,6 +10,9 @@ with every matrix addressed from its full-matrix origin, the sub-block at
 ``offset = rows * col0 + row0``; the launchers here address the bounding box's corner, so the
 tests pass that offset as ``extraOffset``.
 
+The sparse-operand suites and the scale-add suites of the reference's tests are restated the
+same way (``SPARSE_SUITES``, ``CSA_SUITES``).
+
     python tests/gemm_suites.py <out_dir>     (build every case's kernels, plain and auto)
 """
 import concurrent.futures
@@ -61,6 +64,112 @@ SUITES = [
        bbox_a=[[0, 0, 56, 9]], bbox_b=[[0, 0, 9, 9]], bbox_c=[[0, 0, 56, 9]],
        alpha=[1.0, 0.0, 2.0], beta=[1.0, 0.0, 1.5]),
 ]
+
+
+# tests/dense_sparse_gemms/testsuites/*.yaml and tests/sparse_dense_gemms/testsuites/*.yaml:
+# 56x9 dense times a 9x9 sparse matrix (and 56x9 sparse times 9x9 dense), alpha = beta = 1,
+# 100 elements; "random" patterns hold 15 % of the entries in column-major order, "full" all of
+# them (tests/dense_sparse_gemms/test_loader.py:20-45). Patterns are drawn with a fixed seed.
+SPARSE_SUITES = [
+  dict(name='A_B_Dense_Sparse', a=(56, 9), b=(9, 9), trans_a=False, trans_b=False, sparse='b',
+       kind='random'),
+  dict(name='A_B_Dense_Sparse_TransA', a=(9, 56), b=(9, 9), trans_a=True, trans_b=False,
+       sparse='b', kind='random'),
+  dict(name='A_B_Sparse_Dense_Full', a=(56, 9), b=(9, 9), trans_a=False, trans_b=False,
+       sparse='a', kind='full'),
+  dict(name='A_B_Sparse_Dense_Random', a=(56, 9), b=(9, 9), trans_a=False, trans_b=False,
+       sparse='a', kind='random'),
+  dict(name='A_B_Sparse_Dense_TransB', a=(56, 9), b=(9, 9), trans_a=False, trans_b=True,
+       sparse='a', kind='random'),
+]
+
+# tests/csa/testsuites/*.yaml: C = alpha A + beta C on a bounding box (alpha-beta-tests.yaml,
+# with_zero_patch_test.yaml), as pointwise statements.
+CSA_SUITES = [
+  dict(name='AlphaBetaTest', a=(9, 10), c=(9, 10), bbox=[0, 0, 9, 10], alpha=[0.5, 1.5],
+       beta=[1.5, 12.0], n=3),
+  dict(name='ZeroPatch', a=(32, 32), c=(56, 56), bbox=[5, 5, 11, 13], alpha=[2.0], beta=[0.0],
+       n=3),
+]
+
+
+def sparse_pattern(shape, kind, seed):
+  import random
+  if kind == 'full':
+    return [(i, j) for j in range(shape[1]) for i in range(shape[0])]
+  rng = random.Random(seed)
+  count = int(0.15 * shape[0] * shape[1])
+  picked = set()
+  while len(picked) < count:
+    picked.add((rng.randint(0, shape[0] - 1), rng.randint(0, shape[1] - 1)))
+  return sorted(picked, key=lambda ij: ij[0] + ij[1] * shape[0])
+
+
+def sparse_cases():
+  return [dict(s, n=100, pattern=sparse_pattern(s[s['sparse']], s['kind'], 7 + i))
+          for i, s in enumerate(SPARSE_SUITES)]
+
+
+def sparse_descriptions(case):
+  from tensorforge_amd.common.basic_types import Addressing
+  from tensorforge_amd.common.matrix.spp import ListSPP
+  from tensorforge_amd.common.matrix.tensor import SubTensor, Tensor
+  from tensorforge_amd.generators.descriptions import GemmDescr
+  def make(key):
+    shape = list(case[key])
+    spp = ListSPP(case['pattern'], shape) if case['sparse'] == key else None
+    return Tensor(shape, Addressing.STRIDED, alias=key.upper(), spp=spp)
+  A, B = make('a'), make('b')
+  m = case['a'][1] if case['trans_a'] else case['a'][0]
+  n = case['b'][0] if case['trans_b'] else case['b'][1]
+  C = Tensor([m, n], Addressing.STRIDED, alias='C')
+  d = GemmDescr(case['trans_a'], case['trans_b'], SubTensor(A), SubTensor(B), SubTensor(C),
+                alpha=1.0, beta=1.0)
+  return [d], {'A': A, 'B': B, 'C': C}
+
+
+def csa_cases():
+  out = []
+  for s in CSA_SUITES:
+    for alpha, beta in itertools.product(s['alpha'], s['beta']):
+      out.append(dict(name=f'{s["name"]}_{len(out)}', a=s['a'], c=s['c'], bbox=s['bbox'],
+                      alpha=alpha, beta=beta, n=s['n']))
+  return out
+
+
+def csa_descriptions(case):
+  from tensorforge_amd.common.basic_types import Addressing
+  from tensorforge_amd.common.matrix.boundingbox import BoundingBox
+  from tensorforge_amd.common.matrix.tensor import SubTensor, Tensor
+  from tensorforge_amd.generators import optree as ot
+  from tensorforge_amd.generators.descriptions import ElementwiseDescr
+  r0, c0, r1, c1 = case['bbox']
+  box = BoundingBox([r0, c0], [r1, c1])
+  A = Tensor(list(case['a']), Addressing.STRIDED, alias='A')
+  C = Tensor(list(case['c']), Addressing.STRIDED, alias='C')
+  a = ot.tensor(SubTensor(A, box), indices=[-1, -2])
+  c = ot.tensor(SubTensor(C, box), indices=[-1, -2])
+  stmt = ot.assign(c, case['alpha'] * a + case['beta'] * c)
+  return [ElementwiseDescr([stmt])], {'A': A, 'C': C}
+
+
+def _generate(descrs, emitter):
+  from tensorforge_amd.common.basic_types import FloatingPointType
+  from tensorforge_amd.common.context import Context
+  from tensorforge_amd.generators.generator import Generator
+  g = Generator(descrs, Context('gfx950', 'hip', FloatingPointType.DOUBLE), emitter=emitter)
+  g.generate()
+  return g
+
+
+def sparse_generator(case, emitter):
+  descrs, tensors = sparse_descriptions(case)
+  return _generate(descrs, emitter), descrs, tensors
+
+
+def csa_generator(case):
+  descrs, tensors = csa_descriptions(case)
+  return _generate(descrs, 'auto'), descrs, tensors
 
 
 def _as_list(v, scalar_types):
@@ -139,9 +248,16 @@ def build(directory=BUILD_DIR, jobs=8):
     for part in range(PARTS):
       gens = [generator(c, emitter)[0] for i, c in enumerate(all_cases) if part_of(i) == part]
       work.append((translation_unit(gens), f'gemm_suites_{emitter}_{part}'))
+  # these chains differ only in which operand is sparse and in its pattern, which the launcher
+  # name does not hash: one object each
+  for emitter in EMITTERS:
+    for i, c in enumerate(sparse_cases()):
+      work.append((translation_unit([sparse_generator(c, emitter)[0]]),
+                   f'sparse_suites_{emitter}_{i}'))
+  work.append((translation_unit([csa_generator(c)[0] for c in csa_cases()]), 'csa_suites'))
   with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
     list(pool.map(lambda w: compile_source(w[0], w[1], directory, keep_source=False), work))
-  return len(all_cases)
+  return len(all_cases) + len(sparse_cases()) * len(EMITTERS) + len(csa_cases())
 
 
 if __name__ == '__main__':

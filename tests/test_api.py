@@ -263,3 +263,20 @@ def test_yateto_frontend_pointwise_statements():
   pointwise = list(routines.values())[1]._generator
   assert pointwise.get_emitted().schedule['kind'] == 'pointwise'
   assert c.indices == [-1, -2] and c.tensor.bbox.sizes() == [8, 3]
+
+
+def test_one_object_refuses_two_kernels_under_one_name():
+  """The name hashes neither the floating type nor the sparsity pattern: two such chains in one
+  translation unit would bind one launcher to the other's kernel, so it is refused."""
+  from tensorforge_amd import workloads as W
+  from tensorforge_amd.common.exceptions import GenerationError
+  from tensorforge_amd.emit.launcher import translation_unit
+  sparse = Generator(W.ader_volume(FloatingPointType.DOUBLE).descrs, ctx(FloatingPointType.DOUBLE))
+  dense = Generator(W.ader_volume(FloatingPointType.DOUBLE, sparse_k=False).descrs,
+                    ctx(FloatingPointType.DOUBLE))
+  for g in (sparse, dense):
+    g.generate()
+  assert sparse.get_base_name() == dense.get_base_name()
+  with pytest.raises(GenerationError):
+    translation_unit([sparse, dense])
+  assert translation_unit([sparse, sparse]).count('extern "C" void launcher_') == 1

@@ -107,3 +107,77 @@ def test_oracle_matches_reference_driver(index):
                      offsets={tensors[k]: offs[k] for k in offs})
   got = oracle_run(descrs, inp)[tensors['C']]
   np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
+
+
+# ---- sparse-operand and scale-add suites ----
+def sparse_host(case, seed):
+  rng = np.random.default_rng(seed)
+  m = case['a'][1] if case['trans_a'] else case['a'][0]
+  n = case['b'][0] if case['trans_b'] else case['b'][1]
+  host = {k: rng.uniform(-1, 1, size=case['n'] * r * c)
+          for k, (r, c) in (('A', case['a']), ('B', case['b']), ('C', (m, n)))}
+  return host, (m, n)
+
+
+def sparse_closed_form(case, host, mn):
+  """op(A) op(B) + C per element, the sparse operand densified from its packed values (the
+  pattern's list order, column-major as the reference's loader sorts it)."""
+  out = host['C'].copy()
+  m, n = mn
+  vol = {k: case[k][0] * case[k][1] for k in ('a', 'b')}
+  for e in range(case['n']):
+    mats = {}
+    for k in ('a', 'b'):
+      r, c = case[k]
+      flat = host[k.upper()][e * vol[k]:(e + 1) * vol[k]]
+      if case['sparse'] == k:
+        dense = np.zeros((r, c))
+        for pos, (i, j) in enumerate(case['pattern']):
+          dense[i, j] = flat[pos]
+      else:
+        dense = flat.reshape(c, r).T
+      mats[k] = dense
+    opA = mats['a'].T if case['trans_a'] else mats['a']
+    opB = mats['b'].T if case['trans_b'] else mats['b']
+    C = out[e * m * n:(e + 1) * m * n].reshape(n, m).T
+    C[:, :] = opA @ opB + C
+  return out
+
+
+@pytest.mark.parametrize('index', range(len(gs.SPARSE_SUITES)))
+def test_sparse_suites_oracle(index):
+  case = gs.sparse_cases()[index]
+  descrs, tensors = gs.sparse_descriptions(case)
+  host, mn = sparse_host(case, index)
+  want = sparse_closed_form(case, host, mn)
+  inp = OracleInputs({tensors[k]: host[k].copy() for k in host}, case['n'])
+  got = oracle_run(descrs, inp)[tensors['C']]
+  np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
+
+
+def csa_host(case, seed):
+  rng = np.random.default_rng(seed)
+  return {k: rng.uniform(-1, 1, size=case['n'] * case[k.lower()][0] * case[k.lower()][1])
+          for k in ('A', 'C')}
+
+
+def csa_closed_form(case, host):
+  out = host['C'].copy()
+  (ra, ca), (rc, cc) = case['a'], case['c']
+  r0, c0, r1, c1 = case['bbox']
+  for e in range(case['n']):
+    A = host['A'][e * ra * ca:(e + 1) * ra * ca].reshape(ca, ra).T
+    C = out[e * rc * cc:(e + 1) * rc * cc].reshape(cc, rc).T
+    C[r0:r1, c0:c1] = case['alpha'] * A[r0:r1, c0:c1] + case['beta'] * C[r0:r1, c0:c1]
+  return out
+
+
+@pytest.mark.parametrize('index', range(5))
+def test_csa_suites_oracle(index):
+  from oracle.pointwise_oracle import run as pw_run
+  case = gs.csa_cases()[index]
+  descrs, tensors = gs.csa_descriptions(case)
+  host = csa_host(case, index)
+  want = csa_closed_form(case, host)
+  got = pw_run(descrs, OracleInputs({tensors[k]: host[k].copy() for k in host}, case['n']))
+  np.testing.assert_allclose(got[tensors['C']], want, rtol=0, atol=1e-15)
