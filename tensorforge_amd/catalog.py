@@ -453,3 +453,17 @@ for _tag, _kw in STREAM_OPTIONS.items():
     register(f'opt_gemm_f64_{_tag}')(lambda kw=_kw: _option_entry(W.gemm(F64), kw))
 register('opt_ader_dense_f64_unrolled')(
   lambda: _option_entry(W.ader_volume(F64, sparse_k=False), dict(stream_loop='never')))
+
+
+@register('t_pw_big')
+def _pw_big_entry():
+  """Pointwise over more points per element than one workgroup pass covers (120 x 60 = 7,200
+  points: the kernel loops over chunks of 1,024), with a pointer-based source."""
+  from .generators import optree as ot
+  from .generators.descriptions import ElementwiseDescr
+  X = _t([120, 60], Addressing.PTR_BASED, alias='X')
+  Y = _t([120, 60], Addressing.STRIDED, alias='Y')
+  x = ot.tensor(X, indices=[-1, -2])
+  y = ot.tensor(Y, indices=[-1, -2])
+  return Entry('', [ElementwiseDescr([ot.assign(y, 2.0 * y + ot.sqrt(abs(x)) - x * y)])], F64,
+               {'X': X, 'Y': Y})

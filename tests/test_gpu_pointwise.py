@@ -13,7 +13,8 @@ from harness import TOL, gpu_run, key_of, make_inputs, normwise_error, oracle_ou
 
 pytestmark = pytest.mark.gpu
 
-POINTWISE = ['pw_update_f64', 'pw_update_f32', 'pw_plastic_f64', 't_pw_types', 't_pw_math_f32']
+POINTWISE = ['pw_update_f64', 'pw_update_f32', 'pw_plastic_f64', 't_pw_types', 't_pw_math_f32',
+             't_pw_big']
 
 
 def _check(entry, kernel, device, n, seed=0, offsets=None, flags=None):
