@@ -467,3 +467,35 @@ def _pw_big_entry():
   y = ot.tensor(Y, indices=[-1, -2])
   return Entry('', [ElementwiseDescr([ot.assign(y, 2.0 * y + ot.sqrt(abs(x)) - x * y)])], F64,
                {'X': X, 'Y': Y})
+
+
+# ---- contraction chains with fused pointwise statements ----
+def _fused(w, statements_of):
+  from .generators.descriptions import ElementwiseDescr
+  descrs = list(w.descrs) + [ElementwiseDescr(statements_of(w))]
+  return Entry('', descrs, w.fp_type, w.tensors, emitter='auto', meta=w.meta)
+
+
+def _clamp_scale(dest_alias, scalar_alias):
+  from .generators import optree as ot
+
+  def statements(w):
+    t = w.tensors[dest_alias]
+    s = Tensor([], Addressing.SCALAR, alias=scalar_alias)
+    w.tensors[scalar_alias] = s
+    d = ot.tensor(t, indices=[-1, -2])
+    return [ot.assign(d, ot.max(d, 0.0) * ot.scalar(s))]
+  return statements
+
+
+def _soft_limit(dest_alias):
+  from .generators import optree as ot
+
+  def statements(w):
+    d = ot.tensor(w.tensors[dest_alias], indices=[-1, -2])
+    return [ot.conditional(abs(d) > 2.0, [ot.assign(d, 2.0 * ot.tanh(d * 0.5))])]
+  return statements
+
+
+register('t_fused_gemm_pw')(lambda: _fused(W.gemm(F64), _clamp_scale('C', 'dt')))
+register('t_fused_ader_pw')(lambda: _fused(W.ader_volume(F64), _soft_limit('Qp')))

@@ -122,6 +122,8 @@ class ChainProgram:
   products: List[Product]
   fp_type: FloatingPointType
   tmp_shapes: Dict[int, Tuple[int, ...]] = field(default_factory=dict)
+  # pointwise statements applied to the chain's destination before it is stored
+  epilogue: object = None
 
   def arg_by_tensor(self, tensor):
     for a in self.args:

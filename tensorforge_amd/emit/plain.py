@@ -39,6 +39,8 @@ def _lit(fp, v):
 
 
 def emit(program: ir.ChainProgram, context, base_name, num_threads, policy_type) -> Emitted:
+  if program.epilogue is not None:
+    raise GenerationError('the plain emitter does not fuse pointwise statements into a chain')
   fp = program.fp_type
   real = _real(fp)
   lanes = max(1, num_threads)

@@ -553,3 +553,19 @@ def _chunk(prog, w, items, P, PB, epb):
           w(f'if (ok{s} && w{c.index}{s}) __builtin_nontemporal_store(c{c.index}{s}, &{addr});')
         else:
           w(f'if (ok{s} && w{c.index}{s}) {addr} = c{c.index}{s};')
+
+
+def check_epilogue(prog: PointwiseProgram, dest, rows, cols):
+  """Statements fuse into a chain's kernel when they touch no tensor but the chain's final
+  destination, at the entry the chain wrote (index ids -1, -2 over exactly the rows ``rows``
+  and columns ``cols`` it produced); scalars, constants and temporaries are free."""
+  if len(prog.cells) > 1:
+    raise GenerationError('fused statements may only access the chain\'s destination')
+  if prog.extents != [rows[1] - rows[0], cols[1] - cols[0]]:
+    raise GenerationError(f'fused statements range over {prog.extents}, the chain writes '
+                          f'{rows[1] - rows[0]} x {cols[1] - cols[0]}')
+  for c in prog.cells:
+    if c.tensor is not dest or c.indices != (-1, -2) or \
+        tuple(c.sub.bbox.lower()) != (rows[0], cols[0]):
+      raise GenerationError('fused statements may only access the chain\'s destination, at '
+                            'the entries it wrote')

@@ -23,6 +23,7 @@ from dataclasses import dataclass, field
 from typing import List, Optional
 
 from ..common.basic_types import Addressing, FloatingPointType
+from ..common.exceptions import GenerationError
 from . import ir
 from .launcher import Emitted, Writer, kernel_params
 from .traffic import element_traffic
@@ -468,6 +469,8 @@ class _Stream:
              'dpp': self.sched.dpp}
     if self.loop:
       sched['loop'] = True
+    if self.prog.epilogue is not None:
+      sched['epilogue'] = 'pointwise'
     if self.dma:
       sched['lds_dma_slots'] = self.ns
     fallback = None
@@ -595,6 +598,7 @@ class _Stream:
     if self.loop:
       self.loop_blocks(acc)
       with w.block('if (active)'):
+        self.epilogue(acc, pl.rows[0], rows)
         for i in range(rows):
           off = f'{(pl.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
           w(f'g_{pl.dest.name}[{off}] = {acc}[{i}];')
@@ -603,6 +607,7 @@ class _Stream:
     if self.dma:
       self.dma_blocks(acc)
       with w.block('if (active)'):
+        self.epilogue(acc, pl.rows[0], rows)
         for i in range(rows):
           off = f'{(pl.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
           w(f'g_{pl.dest.name}[{off}] = {acc}[{i}];')
@@ -635,6 +640,7 @@ class _Stream:
       self.post_stages(acc)
       return
     with w.block('if (active)'):
+      self.epilogue(acc, pl.rows[0], rows)
       for i in range(rows):
         off = f'{(pl.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
         w(f'g_{pl.dest.name}[{off}] = {acc}[{i}];')
@@ -705,9 +711,35 @@ class _Stream:
     o0, o1 = dp.dest_origin
     s0, s1 = dp.dest_strides
     with w.block('if (active)'):
+      self.epilogue(prev, st.rows[0], st.rows[1] - st.rows[0])
       for i in range(st.rows[1] - st.rows[0]):
         off = f'{(st.rows[0] + i - o0) * s0} + (col + {pl.cols[0] - o1}) * {s1}'
         w(f'g_{pl.dest.name}[{off}] = {prev}[{i}];')
+
+  def epilogue(self, acc, row_lo, nrows):
+    """Pointwise statements fused into the chain (``program.epilogue``): run on the lane's
+    destination column in registers, row by row, before it is stored (active lanes only)."""
+    pw = self.prog.epilogue
+    if pw is None:
+      return
+    from . import pointwise as pwm
+    pwm.check_epilogue(pw, self.pl.dest.tensor, (row_lo, row_lo + nrows),
+                       (self.pl.cols[0], self.pl.cols[0] + self.q))
+    w = self.w
+    cell = pw.cells[0] if pw.cells else None
+    for i in range(nrows):
+      sfx = f'_r{i}'
+      with w.block(''):
+        if cell is not None:
+          w(f'{cell.dtype} c{cell.index}{sfx} = {acc}[{i}];')
+          w(f'bool w{cell.index}{sfx} = false;')
+          w(f'(void)w{cell.index}{sfx};')
+        for t in pw.temps:
+          dt = pw.fp_type if t.datatype is None else t.datatype
+          w(f'{dt} {t.name}{sfx} = {dt.literal(0)};')
+        pwm._Code(pw, w, sfx).statements(pw.statements)
+        if cell is not None:
+          w(f'{acc}[{i}] = c{cell.index}{sfx};')
 
   # ---- cross-group prefetch pipeline (unrolled blocks) ----
   def _pipe_steps(self, nr):
@@ -1451,6 +1483,8 @@ def emit(program, context, base_name, schedule, pl) -> Emitted:
   passes = _row_passes(pl, schedule)
   if len(passes) == 1:
     return _Stream(program, context, base_name, schedule, pl).emit()
+  if program.epilogue is not None:
+    raise GenerationError('pointwise statements do not fuse into a chain run in row bands')
   # several passes over the contraction, each for a band of destination rows: the accumulator
   # of a band fits the registers and the band's shared rows fit LDS; the streamed operand is
   # read once per pass

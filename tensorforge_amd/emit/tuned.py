@@ -848,6 +848,8 @@ def emit(program, context, base_name, schedule=None) -> Emitted:
       emitted = stream.emit(program, context, base_name, sched, pl)
       if emitted.lds_bytes <= LDS_LIMIT:
         return emitted
+  if program.epilogue is not None:
+    raise GenerationError('pointwise statements fuse only into a streaming chain')
   emitted = _Emitter(program, context, base_name, schedule).emit()
   if emitted.lds_bytes > LDS_LIMIT and schedule.waves_per_block > 1:
     # fewer wavefronts per workgroup: each wavefront owns its LDS slice

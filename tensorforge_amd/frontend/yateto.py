@@ -8,6 +8,7 @@ duck-typed yateto objects (``op.name``, ``op.memoryLayout``, ``op.eqspp``, ``op.
 from ..common.aux import generate_tmp_tensor
 from ..common.basic_types import Addressing, DataFlowDirection, FloatingPointType
 from ..common.context import Context
+from ..common.exceptions import GenerationError
 from ..common.matrix.boundingbox import BoundingBox as BBox
 from ..common.matrix.spp import FullSPP, ListSPP
 from ..common.matrix.tensor import SubTensor, Tensor
@@ -203,10 +204,10 @@ class GpuKernelGeneratorV1:
 
   # ---- code generation ----
   def generate(self, cpp, routine_cache):
-    """One kernel and one launcher call per run of descriptions of one kind: contraction chains
-    (``MultilinearDescr``) and pointwise statements (``ElementwiseDescr``) become separate
-    kernels launched in order on the same stream (the reference puts both in one kernel). A chain
-    temporary lives inside one kernel: using it across the split raises at generation."""
+    """One kernel for the whole list where it fuses (a contraction chain followed by pointwise
+    statements on its destination, as the reference generates them); otherwise one kernel and
+    one launcher call per run of descriptions of one kind, launched in order on the same stream.
+    A chain temporary lives inside one kernel: using it across a split raises at generation."""
     fptype = FloatingPointType.str2enum(self._arch.typename) if hasattr(self._arch, 'typename') \
       else FloatingPointType.FLOAT
     segments = []
@@ -216,14 +217,28 @@ class GpuKernelGeneratorV1:
         segments[-1][1].append(d)
       else:
         segments.append((pointwise, [d]))
+    if len(segments) == 2 and not segments[0][0]:
+      context = Context(arch=self._arch.name, backend=self._arch.backend, fp_type=fptype)
+      generator = TensorForgeGenerator(list(self._descr_list), context)
+      generator.register()
+      try:
+        generator.generate()
+      except GenerationError:
+        generator = None
+      if generator is not None:
+        self._emit(cpp, routine_cache, generator, context)
+        return
     for _, descrs in segments:
       context = Context(arch=self._arch.name, backend=self._arch.backend, fp_type=fptype)
       generator = TensorForgeGenerator(descrs, context)
       generator.register()
-      cpp(self._gen_call_site(generator))
-      name = generator.get_base_name()
       generator.generate()
-      routine_cache.addRoutine(name, TensorForgeWriter(generator, context.get_vm().get_headers()))
+      self._emit(cpp, routine_cache, generator, context)
+
+  def _emit(self, cpp, routine_cache, generator, context):
+    cpp(self._gen_call_site(generator))
+    routine_cache.addRoutine(generator.get_base_name(),
+                             TensorForgeWriter(generator, context.get_vm().get_headers()))
 
   def _gen_call_site(self, generator):
     mat_name_map, offset_name_map = {}, {}

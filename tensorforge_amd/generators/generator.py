@@ -142,10 +142,11 @@ class Generator:
     if not self._is_registerd:
       self.register()
     pointwise = [isinstance(d, ElementwiseDescr) for d in self.descr_list]
-    if any(pointwise) and not all(pointwise):
-      raise GenerationError('a kernel is either pointwise (ElementwiseDescr only) or a '
-                            'contraction chain (MultilinearDescr only); generate them as two '
-                            'kernels')
+    tail = pointwise.index(True) if any(pointwise) else len(pointwise)
+    fused = 0 < tail < len(pointwise) and all(pointwise[tail:])
+    if any(pointwise) and not all(pointwise) and not fused:
+      raise GenerationError('pointwise statements (ElementwiseDescr) come after the contraction '
+                            'chain they fuse into, or stand alone')
     self._deduce_num_threads()
     from ..emit import plain, tuned, launcher
     if pointwise and all(pointwise):
@@ -159,7 +160,15 @@ class Generator:
       self._launcher = launcher.launcher_source(self, emitted)
       self._header = f'{self._generate_launcher_proto(with_defaults=True)};\n'
       return
-    self._program = ir_mod.lower(self.descr_list, self._global_tensors, self._context.fp_type)
+    self._program = ir_mod.lower(self.descr_list[:tail], self._global_tensors,
+                                 self._context.fp_type)
+    if fused:
+      # the statements run on the chain's destination in registers, before it is stored
+      from ..emit import pointwise as pw
+      self._program.epilogue = pw.lower(self.descr_list[tail:], self._global_tensors,
+                                        self._context.fp_type)
+      if self._emitter_choice not in ('auto', 'tuned'):
+        raise GenerationError(f'{self._emitter_choice} does not fuse pointwise statements')
     choice = self._emitter_choice
     emitted = None
     if choice in ('mfma_fp32', 'mfma_bf16x3'):

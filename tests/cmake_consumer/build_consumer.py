@@ -4,9 +4,10 @@
 
 Generation goes through the yateto frontend the way yateto's code generator drives it:
 ``add_tensor`` records, an ``add_operation`` multilinear record (C = A * B), ``add_scalar``
-pointwise statements (C = max(C, 0) * dt), then ``generate(cpp, routine_cache)``, whose writers
-append each kernel and launcher to the routine file (``kernels.cpp``) and return the prototypes
-(``kernels.h``, with the emitted call sites wrapped in ``tf_execute``). CMake then builds
+pointwise statements (C = max(C, 0) * dt, fused into the chain's kernel), then
+``generate(cpp, routine_cache)``, whose writers append the kernel and launcher to the routine
+file (``kernels.cpp``) and return the prototype (``kernels.h``, with the emitted call site
+wrapped in ``tf_execute``). CMake then builds
 ``consumer`` against ``find_package(TensorForge)``.
 """
 import os

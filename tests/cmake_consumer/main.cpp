@@ -1,6 +1,6 @@
-// C++ consumer of a generated routine file: C_e = A * B_e (a contraction kernel), then
-// C_e = max(C_e, 0) * dt (a pointwise kernel), launched through the call sites the yateto
-// frontend emitted (tf_execute in kernels.h). Checked against a plain CPU loop.
+// C++ consumer of a generated routine file: C_e = A * B_e followed by C_e = max(C_e, 0) * dt
+// (one kernel: the pointwise statements fused into the contraction), launched through the call
+// site the yateto frontend emitted (tf_execute in kernels.h). Checked against a plain CPU loop.
 // Exit status: 0 agreement, 1 disagreement, 2 a device or launch error.
 #include <hip/hip_runtime.h>
 

@@ -82,11 +82,13 @@ def oracle_outputs(entry, host, num_elements, offsets=None, flags=None):
         (np.asarray(v, dtype=np.float64).copy() if not isinstance(v, (float, int)) else v)
   offs = {t: (offsets or {}).get(key_of(t), 0) for t in gen._global_tensors}
   inp = OracleInputs(arrays, num_elements, flags=flags, offsets=offs)
-  if is_pointwise(entry):
+  chain = [d for d in entry.descrs if type(d).__name__ != 'ElementwiseDescr']
+  statements = [d for d in entry.descrs if type(d).__name__ == 'ElementwiseDescr']
+  out = oracle_run(chain, inp) if chain else inp.arrays
+  if statements:
+    # a chain's fused statements run after the chain, on what it wrote
     from oracle.pointwise_oracle import run as pointwise_run
-    out = pointwise_run(entry.descrs, inp)
-  else:
-    out = oracle_run(entry.descrs, inp)
+    out = pointwise_run(statements, inp)
   return {key_of(t): out[t] for t in gen._global_tensors if t in out}
 
 

@@ -257,12 +257,33 @@ def test_yateto_frontend_pointwise_statements():
       routines[name] = writer
 
   fe.generate(lines.append, Cache())
-  assert len(routines) == 2 and len(lines) == 2
+  # the statements act on the chain's destination only: one kernel, as the reference fuses them
+  assert len(routines) == 1 and len(lines) == 1
   assert lines[0].startswith('launcher_kernel_') and 'const_cast<const double*>(A)' in lines[0]
-  assert lines[1].endswith('(C, extraOffset_C, dt, numElements, flags, streamPtr);')
+  assert lines[0].endswith('dt, numElements, flags, streamPtr);')
+  fused = list(routines.values())[0]._generator
+  assert fused.get_emitted().schedule.get('epilogue') == 'pointwise'
+  assert c.indices == [-1, -2] and c.tensor.bbox.sizes() == [8, 3]
+
+  # statements that read another tensor do not fuse: two kernels, called in order
+  fe = fe_cls(Arch())
+  for name, shape, addr in (('A', [8, 8], '&'), ('B', [8, 3], 'n*N+o&'), ('C', [8, 3], 'n*N+o&')):
+    fe.add_tensor({'name': name, 'datatype': 'f64', 'addressing': addr,
+                   'storage': {'shape': shape, 'type': 'full'}, 'values': None,
+                   'flags': {'temporary': False, 'constant': False}})
+  fe.add_operation({'type': 'multilinear', 'result': ref('C'), 'args': [ref('A'), ref('B')],
+                    'condition': [], 'target': [[0, -1], [-1, 1]], 'permute': [[0, 1], [0, 1]],
+                    'linear': {'alpha': None, 'add': False}})
+  B = YTensor('B', [8, 3])
+  c2, b2 = ot.tensor(C), ot.tensor(B)
+  fe.add_scalar([C, B], [ot.assign(c2, c2 + b2)], [[-1, -2], [-1, -2]])
+  lines, routines = [], {}
+  fe.generate(lines.append, Cache())
+  assert len(routines) == 2 and len(lines) == 2
+  assert lines[1].endswith('(C, extraOffset_C, const_cast<const double*>(B), extraOffset_B, '
+                           'numElements, flags, streamPtr);')
   pointwise = list(routines.values())[1]._generator
   assert pointwise.get_emitted().schedule['kind'] == 'pointwise'
-  assert c.indices == [-1, -2] and c.tensor.bbox.sizes() == [8, 3]
 
 
 def test_one_object_refuses_two_kernels_under_one_name():

@@ -19,15 +19,15 @@ def test_cmake_path_holds_the_package():
 
 
 def test_routine_file_and_call_sites(tmp_path):
-  """Two kernels appended to one routine file (contraction, then pointwise), their extern "C"
-  prototypes, and call sites in order; the file compiles for gfx950 on its own."""
+  """The chain and its pointwise statements as one fused kernel appended to the routine file,
+  its extern "C" prototype and call site; the file compiles for gfx950 on its own."""
   import build_consumer
   calls = build_consumer.generate(str(tmp_path))
-  assert len(calls) == 2
-  assert 'const_cast<const double*>(A)' in calls[0] and calls[1].endswith(
-    '(C, extraOffset_C, dt, numElements, flags, streamPtr);')
+  assert len(calls) == 1
+  assert 'const_cast<const double*>(A)' in calls[0] and calls[0].endswith(
+    'dt, numElements, flags, streamPtr);')
   text = (tmp_path / 'kernels.h').read_text()
-  assert text.count('extern "C" void launcher_kernel_') == 2
+  assert text.count('extern "C" void launcher_kernel_') == 1
   src = tmp_path / 'kernels.cpp'
   from tensorforge_amd.runtime.build import INCLUDE_DIR, hipcc, hipcc_flags
   flags = [f for f in hipcc_flags() if f not in ('-shared', '-fPIC')]

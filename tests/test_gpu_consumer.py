@@ -1,5 +1,5 @@
-"""The CMake-built C++ consumer (tests/cmake_consumer): a contraction kernel and a pointwise
-kernel from one generated routine file, called through their C ABI from C++, checked inside the
+"""The CMake-built C++ consumer (tests/cmake_consumer): a contraction with fused pointwise
+statements from a generated routine file, called through its C ABI from C++, checked inside the
 program against a CPU loop (normwise per element <= 1e-12)."""
 import os
 import subprocess

@@ -102,13 +102,13 @@ def test_generation_errors():
   d = ElementwiseDescr([ot.assign(ot.tensor(V, indices=[-1]), ot.tensor(Y, indices=[-1, -2]))])
   with pytest.raises(GenerationError):
     _gen([d])
-  # pointwise and contraction descriptions do not share a kernel
+  # statements come after the chain they fuse into, never before it
   A = Tensor([4, 4], Addressing.NONE, alias='A')
   m = MultilinearDescr(SubTensor(Y), [SubTensor(A), SubTensor(X)], [[0, -1], [-1, 1]],
                        [[0, 1], [0, 1]])
   e = ElementwiseDescr([ot.assign(ot.tensor(Y, indices=[-1, -2]), 1.0)])
   with pytest.raises(GenerationError):
-    _gen([m, e])
+    _gen([e, m])
   # sparse operands
   S = Tensor([4, 4], Addressing.STRIDED, spp=ListSPP([(0, 0), (1, 1)], (4, 4)), alias='S')
   d = ElementwiseDescr([ot.assign(ot.tensor(Y, indices=[-1, -2]), ot.tensor(S, indices=[-1, -2]))])
@@ -292,3 +292,58 @@ def test_empty_batch_and_flags_leave_everything():
   inp.num_elements = 0
   q0 = inp.arrays[Q].copy()
   np.testing.assert_array_equal(pw_run(e.descrs, inp)[Q], q0)
+
+
+# ---- statements fused into a contraction chain ----
+def test_fused_statements_generate_one_kernel():
+  g = catalog.get('t_fused_gemm_pw').generator
+  g.generate()
+  em = g.get_emitted()
+  assert em.schedule.get('epilogue') == 'pointwise' and em.emitter == 'tuned'
+  assert 'fmax(' in g.get_kernel()
+  # the scalar of the statements is a launcher argument after the chain's operands
+  assert [t for t, _ in g.launcher_signature()][:6] == ['double*', 'unsigned', 'double*',
+                                                         'double*', 'unsigned', 'double']
+
+
+def test_fusion_rules():
+  from tensorforge_amd import workloads as W
+  w = W.gemm(F64)
+  C, B = w.tensors['C'], w.tensors['B']
+  other = ot.tensor(B, indices=[-1, -2])
+  c = ot.tensor(C, indices=[-1, -2])
+  # statements reading another tensor do not fuse
+  with pytest.raises(GenerationError):
+    _gen(list(w.descrs) + [ElementwiseDescr([ot.assign(c, c + other)])])
+  # statements before the chain
+  w = W.gemm(F64)
+  c = ot.tensor(w.tensors['C'], indices=[-1, -2])
+  with pytest.raises(GenerationError):
+    _gen([ElementwiseDescr([ot.assign(c, c * 2.0)])] + list(w.descrs))
+  # the plain emitter does not fuse
+  w = W.gemm(F64)
+  c = ot.tensor(w.tensors['C'], indices=[-1, -2])
+  g = Generator(list(w.descrs) + [ElementwiseDescr([ot.assign(c, c * 2.0)])],
+                Context('gfx950', 'hip', F64), emitter='plain')
+  with pytest.raises(GenerationError):
+    g.generate()
+
+
+def test_fused_oracle_closed_form():
+  """The oracle runs the chain, then the statements on what it wrote: max(C + A B, 0) * dt."""
+  import sys
+  import os
+  sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+  from harness import make_inputs, oracle_outputs
+  e = catalog.get('t_fused_gemm_pw')
+  e.generator.register()
+  n = 4
+  host = make_inputs(e, n, seed=1)
+  out = oracle_outputs(e, host, n)
+  A = host['A'].reshape(56, 56).T
+  for el in range(n):
+    B = host['B'][el * 504:(el + 1) * 504].reshape(9, 56).T
+    C = host['C'][el * 504:(el + 1) * 504].reshape(9, 56).T
+    want = np.maximum(C + A @ B, 0.0) * host['dt']
+    got = out['C'][el * 504:(el + 1) * 504].reshape(9, 56).T
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
