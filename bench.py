@@ -359,6 +359,19 @@ def pointwise_leg(device, steps, warmup, n=262144):
     res[name] = {'value': n / t, 'ms_per_step': t * 1e3, 'bytes_per_element': bpe,
                  'hbm_GBps': bpe * n / t / 1e9, 'frac_of_peak': bpe * n / t / 1e9 / HBM_PEAK_GBPS}
     del bufs
+  # fusion: C = A B followed by C = max(C, 0) * dt as one kernel, against the two kernels
+  _, fused = catalog.load('t_fused_gemm_pw')
+  _, gemm = catalog.load('gemm_f64_tuned')
+  _, clamp = catalog.load('pw_clamp_scale_f64')
+  bufs = synthetic_buffers(fused.generator, n, device)
+  bufs.pop('__keep__')
+  t_fused = time_kernel(fused, bufs, n, steps, warmup)
+  t_gemm = time_kernel(gemm, bufs, n, steps, warmup)
+  t_clamp = time_kernel(clamp, bufs, n, steps, warmup)
+  res['fused_gemm_statements'] = {'fused_ms': t_fused * 1e3, 'gemm_ms': t_gemm * 1e3,
+                                  'statements_ms': t_clamp * 1e3,
+                                  'split_over_fused': (t_gemm + t_clamp) / t_fused}
+  del bufs
   res['unit'] = 'elements/s'
   res['elements'] = n
   return res

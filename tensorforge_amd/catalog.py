@@ -499,3 +499,15 @@ def _soft_limit(dest_alias):
 
 register('t_fused_gemm_pw')(lambda: _fused(W.gemm(F64), _clamp_scale('C', 'dt')))
 register('t_fused_ader_pw')(lambda: _fused(W.ader_volume(F64), _soft_limit('Qp')))
+
+
+@register('pw_clamp_scale_f64')
+def _pw_clamp_scale_entry():
+  """C = max(C, 0) * dt alone (the statements of t_fused_gemm_pw), for timing the fusion."""
+  from .generators import optree as ot
+  from .generators.descriptions import ElementwiseDescr
+  C = _t([56, 9], Addressing.STRIDED, alias='C')
+  dt = _t([], Addressing.SCALAR, alias='dt')
+  c = ot.tensor(C, indices=[-1, -2])
+  return Entry('', [ElementwiseDescr([ot.assign(c, ot.max(c, 0.0) * ot.scalar(dt))])], F64,
+               {'C': C, 'dt': dt})
