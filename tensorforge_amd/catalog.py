@@ -511,3 +511,18 @@ def _pw_clamp_scale_entry():
   c = ot.tensor(C, indices=[-1, -2])
   return Entry('', [ElementwiseDescr([ot.assign(c, ot.max(c, 0.0) * ot.scalar(dt))])], F64,
                {'C': C, 'dt': dt})
+
+
+def _time_update(dest_alias, source_alias, scalar_alias):
+  from .generators import optree as ot
+
+  def statements(w):
+    s = Tensor([], Addressing.SCALAR, alias=scalar_alias)
+    w.tensors[scalar_alias] = s
+    d = ot.tensor(w.tensors[dest_alias], indices=[-1, -2])
+    q = ot.tensor(w.tensors[source_alias], indices=[-1, -2])
+    return [ot.assign(d, q + ot.scalar(s) * d)]
+  return statements
+
+
+register('t_fused_ader_update')(lambda: _fused(W.ader_volume(F64), _time_update('Qp', 'Q', 'dt')))

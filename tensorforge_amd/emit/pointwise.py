@@ -556,16 +556,30 @@ def _chunk(prog, w, items, P, PB, epb):
 
 
 def check_epilogue(prog: PointwiseProgram, dest, rows, cols):
-  """Statements fuse into a chain's kernel when they touch no tensor but the chain's final
-  destination, at the entry the chain wrote (index ids -1, -2 over exactly the rows ``rows``
-  and columns ``cols`` it produced); scalars, constants and temporaries are free."""
-  if len(prog.cells) > 1:
-    raise GenerationError('fused statements may only access the chain\'s destination')
+  """Statements fuse into a chain's kernel when they range over exactly the rows ``rows`` and
+  columns ``cols`` the chain produced, write nothing but the chain's final destination at the
+  entries it wrote (index ids -1, -2), and read other tensors only through arguments (a point
+  is a destination row of the lane's column); scalars and temporaries are free."""
   if prog.extents != [rows[1] - rows[0], cols[1] - cols[0]]:
     raise GenerationError(f'fused statements range over {prog.extents}, the chain writes '
                           f'{rows[1] - rows[0]} x {cols[1] - cols[0]}')
   for c in prog.cells:
-    if c.tensor is not dest or c.indices != (-1, -2) or \
-        tuple(c.sub.bbox.lower()) != (rows[0], cols[0]):
-      raise GenerationError('fused statements may only access the chain\'s destination, at '
-                            'the entries it wrote')
+    if c.tensor is dest:
+      if c.indices != (-1, -2) or tuple(c.sub.bbox.lower()) != (rows[0], cols[0]):
+        raise GenerationError('fused statements access the chain\'s destination only at the '
+                              'entries it wrote')
+    elif c.written:
+      raise GenerationError('fused statements write nothing but the chain\'s destination')
+    elif c.arg.kind != SRC_GLOBAL:
+      raise GenerationError(f'{c.tensor.name}: constant tables are not read by fused statements')
+
+
+def epilogue_address(cell: Cell, n) -> str:
+  """Offset of ``cell``'s entry at point ``n`` (C expressions) from its element's base."""
+  t = cell.tensor
+  strides = colmajor_strides(t.shape)
+  terms = [str(sum((lo - o) * st for lo, o, st in
+                   zip(cell.sub.bbox.lower(), t.bbox.lower(), strides)))]
+  for dim, idx in enumerate(cell.indices):
+    terms.append(f'({n[-idx - 1]}) * {strides[dim]}')
+  return ' + '.join(terms)

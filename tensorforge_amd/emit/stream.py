@@ -726,20 +726,27 @@ class _Stream:
     pwm.check_epilogue(pw, self.pl.dest.tensor, (row_lo, row_lo + nrows),
                        (self.pl.cols[0], self.pl.cols[0] + self.q))
     w = self.w
-    cell = pw.cells[0] if pw.cells else None
+    dest = self.pl.dest.tensor
     for i in range(nrows):
       sfx = f'_r{i}'
       with w.block(''):
-        if cell is not None:
-          w(f'{cell.dtype} c{cell.index}{sfx} = {acc}[{i}];')
-          w(f'bool w{cell.index}{sfx} = false;')
-          w(f'(void)w{cell.index}{sfx};')
+        for c in pw.cells:
+          if c.tensor is dest:
+            w(f'{c.dtype} c{c.index}{sfx} = {acc}[{i}];')
+            w(f'bool w{c.index}{sfx} = false;')
+            w(f'(void)w{c.index}{sfx};')
+          else:
+            # another argument, read at this point: row i of the window, the lane's column
+            off = pwm.epilogue_address(c, [str(i), 'col'])
+            base = f'g_{c.arg.name}' if c.arg.per_element else c.arg.name
+            w(f'const {c.dtype} c{c.index}{sfx} = ({c.dtype}){base}[{off}];')
         for t in pw.temps:
           dt = pw.fp_type if t.datatype is None else t.datatype
           w(f'{dt} {t.name}{sfx} = {dt.literal(0)};')
         pwm._Code(pw, w, sfx).statements(pw.statements)
-        if cell is not None:
-          w(f'{acc}[{i}] = c{cell.index}{sfx};')
+        for c in pw.cells:
+          if c.tensor is dest:
+            w(f'{acc}[{i}] = c{c.index}{sfx};')
 
   # ---- cross-group prefetch pipeline (unrolled blocks) ----
   def _pipe_steps(self, nr):

@@ -265,7 +265,7 @@ def test_yateto_frontend_pointwise_statements():
   assert fused.get_emitted().schedule.get('epilogue') == 'pointwise'
   assert c.indices == [-1, -2] and c.tensor.bbox.sizes() == [8, 3]
 
-  # statements that read another tensor do not fuse: two kernels, called in order
+  # statements that write another tensor do not fuse: two kernels, called in order
   fe = fe_cls(Arch())
   for name, shape, addr in (('A', [8, 8], '&'), ('B', [8, 3], 'n*N+o&'), ('C', [8, 3], 'n*N+o&')):
     fe.add_tensor({'name': name, 'datatype': 'f64', 'addressing': addr,
@@ -276,12 +276,12 @@ def test_yateto_frontend_pointwise_statements():
                     'linear': {'alpha': None, 'add': False}})
   B = YTensor('B', [8, 3])
   c2, b2 = ot.tensor(C), ot.tensor(B)
-  fe.add_scalar([C, B], [ot.assign(c2, c2 + b2)], [[-1, -2], [-1, -2]])
+  fe.add_scalar([C, B], [ot.assign(b2, c2 + b2)], [[-1, -2], [-1, -2]])
   lines, routines = [], {}
   fe.generate(lines.append, Cache())
   assert len(routines) == 2 and len(lines) == 2
-  assert lines[1].endswith('(C, extraOffset_C, const_cast<const double*>(B), extraOffset_B, '
-                           'numElements, flags, streamPtr);')
+  assert lines[1].endswith('(C, extraOffset_C, B, extraOffset_B, numElements, flags, '
+                           'streamPtr);')
   pointwise = list(routines.values())[1]._generator
   assert pointwise.get_emitted().schedule['kind'] == 'pointwise'
 

@@ -14,7 +14,7 @@ from harness import TOL, gpu_run, key_of, make_inputs, normwise_error, oracle_ou
 pytestmark = pytest.mark.gpu
 
 POINTWISE = ['pw_update_f64', 'pw_update_f32', 'pw_plastic_f64', 't_pw_types', 't_pw_math_f32',
-             't_pw_big', 't_fused_gemm_pw', 't_fused_ader_pw']
+             't_pw_big', 't_fused_gemm_pw', 't_fused_ader_pw', 't_fused_ader_update']
 
 
 def _check(entry, kernel, device, n, seed=0, offsets=None, flags=None):
@@ -127,7 +127,7 @@ def test_plastic_full_size_invariants(device):
   assert torch.equal(after[:, 6:], before[:, 6:])
 
 
-@pytest.mark.parametrize('name', ['t_fused_gemm_pw', 't_fused_ader_pw'])
+@pytest.mark.parametrize('name', ['t_fused_gemm_pw', 't_fused_ader_pw', 't_fused_ader_update'])
 def test_fused_statements_over_many_rounds(name, device):
   """Chains with fused statements at 60,001 elements (several element groups per workgroup)
   against the oracle (chain, then statements) on a sample and with flags."""

@@ -312,9 +312,14 @@ def test_fusion_rules():
   C, B = w.tensors['C'], w.tensors['B']
   other = ot.tensor(B, indices=[-1, -2])
   c = ot.tensor(C, indices=[-1, -2])
-  # statements reading another tensor do not fuse
+  # statements writing another tensor do not fuse (reading one does)
   with pytest.raises(GenerationError):
-    _gen(list(w.descrs) + [ElementwiseDescr([ot.assign(c, c + other)])])
+    _gen(list(w.descrs) + [ElementwiseDescr([ot.assign(other, c + other)])])
+  w = W.gemm(F64)
+  c = ot.tensor(w.tensors['C'], indices=[-1, -2])
+  g = _gen(list(w.descrs) + [ElementwiseDescr([ot.assign(c, c + ot.tensor(w.tensors['B'],
+                                                                        indices=[-1, -2]))])])
+  assert g.get_emitted().schedule.get('epilogue') == 'pointwise'
   # statements before the chain
   w = W.gemm(F64)
   c = ot.tensor(w.tensors['C'], indices=[-1, -2])
