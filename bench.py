@@ -384,7 +384,8 @@ def tuner_leg():
   from tensorforge_amd.tune import model as M
   path = os.path.join(ROOT, 'tensorforge_amd', 'tune', 'data', 'timings_mi355x.jsonl')
   recs = M.load(path)
-  s = M.cross_validate(recs, folds=5, seed=0, prune_k=3, epochs=800, centered=True)
+  s = M.cross_validate(recs, folds=5, seed=0, prune_k=3, epochs=400, centered=True,
+                       ensemble=5)
   s['candidates_per_chain'] = len(recs) / s['chains']
   return s
 

@@ -55,12 +55,29 @@ def arrays(records, centered=True):
   return X, y
 
 
-def train(records, epochs=400, lr=2e-3, seed=0, device='cuda:0', centered=True):
+class Ensemble:
+  """Several networks trained from different seeds; the prediction is their mean (lower
+  variance than one network: the ranking of close candidates is what decides a pick)."""
+
+  def __init__(self, models):
+    self.models = list(models)
+
+  def predict(self, X):
+    return np.mean([m.predict(X) for m in self.models], axis=0)
+
+
+def train(records, epochs=400, lr=2e-3, seed=0, device='cuda:0', centered=True, ensemble=1):
+  """A network (``ensemble`` = 1) or an :class:`Ensemble` of ``ensemble`` networks trained on
+  ``records``; returns it and the training-loss history of the first network."""
   from .mlp import CostMLP
   X, y = arrays(records, centered)
-  model = CostMLP(X.shape[1], seed=seed, device=device)
-  history = model.fit(X, y, epochs=epochs, lr=lr, batch=128, seed=seed)
-  return model, history
+  models, history = [], None
+  for k in range(ensemble):
+    model = CostMLP(X.shape[1], seed=seed + 1000 * k, device=device)
+    h = model.fit(X, y, epochs=epochs, lr=lr, batch=128, seed=seed + 1000 * k)
+    history = history if history is not None else h
+    models.append(model)
+  return (models[0] if ensemble == 1 else Ensemble(models)), history
 
 
 def default_choice(labels):
@@ -112,9 +129,10 @@ def pruned(order, labels, k):
 
 
 def cross_validate(records, folds=4, seed=0, prune_k=2, epochs=400, device='cuda:0',
-                   centered=True):
-  """Every chain held out once: ``folds`` models, each trained on the other chains' timings;
-  the per-chain slowdowns of all folds are pooled into one summary."""
+                   centered=True, ensemble=1):
+  """Every chain held out once: ``folds`` models (each an ensemble of ``ensemble`` networks),
+  each trained on the other chains' timings; the per-chain slowdowns of all folds are pooled
+  into one summary."""
   chains = sorted(by_chain(records))
   rng = np.random.default_rng(seed)
   rng.shuffle(chains)
@@ -124,7 +142,8 @@ def cross_validate(records, folds=4, seed=0, prune_k=2, epochs=400, device='cuda
   for i, test in enumerate(parts):
     train_recs = [r for r in records if r['chain'] not in test]
     test_recs = [r for r in records if r['chain'] in test]
-    model, _ = train(train_recs, epochs=epochs, seed=seed + i, device=device, centered=centered)
+    model, _ = train(train_recs, epochs=epochs, seed=seed + i, device=device, centered=centered,
+                     ensemble=ensemble)
     pooled.append((model, test_recs))
   out = {'model_pick': [], 'pruned_search': [], 'default': [], 'random_mean': []}
   for model, recs in pooled:
@@ -137,7 +156,8 @@ def cross_validate(records, folds=4, seed=0, prune_k=2, epochs=400, device='cuda
       out['pruned_search'].append(times[pruned(order, labels, prune_k)].min() / best - 1)
       out['default'].append(times[labels.index(default_choice(labels))] / best - 1)
       out['random_mean'].append(float(np.mean(times / best - 1)))
-  summary = {'chains': len(out['model_pick']), 'folds': folds, 'prune_k': prune_k}
+  summary = {'chains': len(out['model_pick']), 'folds': folds, 'prune_k': prune_k,
+             'ensemble': ensemble, 'epochs': epochs}
   for key, v in out.items():
     v = np.array(v)
     summary[key] = {'mean_slowdown': float(v.mean()), 'median_slowdown': float(np.median(v)),

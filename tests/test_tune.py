@@ -60,3 +60,20 @@ def test_evaluation_measures_slowdown_against_the_sweep():
   assert s['default']['mean_slowdown'] > 0.5
   train, test = M.split(recs, 0.25, seed=1)
   assert {r['chain'] for r in train}.isdisjoint({r['chain'] for r in test})
+
+
+def test_ensemble_predicts_the_mean_of_its_networks():
+  """The ensemble's score is the mean of its members' scores, so its pick can differ from
+  every member's (here member one prefers candidate 0, member two candidate 2, the mean 1)."""
+  class Fixed:
+    def __init__(self, v):
+      self.v = np.array(v, dtype=float)
+
+    def predict(self, X):
+      return self.v[:len(X)]
+  ens = M.Ensemble([Fixed([0.0, 1.0, 3.0]), Fixed([3.0, 1.0, 0.0])])
+  np.testing.assert_allclose(ens.predict(np.zeros((3, 2))), [1.5, 1.0, 1.5])
+  recs = [{'chain': 'c', 'label': lab, 'time': t, 'features': [0.0, 0.0]}
+          for lab, t in (('plain', 3.0), ('tuned/a', 1.0), ('tuned/b', 2.0))]
+  s = M.evaluate(ens, recs, prune_k=1)
+  assert s['model_pick']['mean_slowdown'] == 0.0
