@@ -1,0 +1,52 @@
+"""Write ``kernel_names.json``: the reference generator's kernel name and launcher header for
+every case of ``name_cases.py``.
+
+Run in a checkout that holds the reference next to this repository (it imports the reference's
+``tensorforge`` package from ``/root/reference``; nothing of the reference is copied, only the
+names and prototypes it produces):
+
+    python tests/golden/make_kernel_names.py
+
+The kernel name is ``kernel_`` + the first 10 hex digits of the md5 of the description string
+(reference ``tensorforge/generators/generator.py:349-362``); the header is the launcher prototype
+(``generator.py:426-436``). The reference is asked for an arch from its own table (``gfx90a``);
+neither the name nor the prototype depends on the arch.
+"""
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REFERENCE = '/root/reference'
+
+
+def main():
+  sys.path.insert(0, REFERENCE)
+  import tensorforge
+  assert os.path.abspath(tensorforge.__file__).startswith(REFERENCE), tensorforge.__file__
+  from tensorforge.common.basic_types import FloatingPointType
+  from tensorforge.common.context import Context
+  from tensorforge.generators.generator import Generator
+  import name_cases
+
+  fp = {'double': FloatingPointType.DOUBLE, 'float': FloatingPointType.FLOAT}
+  records = []
+  for label, fp_name, descrs in name_cases.cases():
+    ctx = Context(arch='gfx90a', backend='hip', fp_type=fp[fp_name])
+    gen = Generator(descrs, ctx)
+    gen.register()
+    rec = {'label': label, 'fp_type': fp_name, 'name': gen.get_base_name(), 'header': None}
+    try:
+      gen.generate()
+      rec['header'] = gen.get_header()
+    except Exception as exc:  # the name is still pinned; say why the prototype is not
+      rec['header_error'] = f'{type(exc).__name__}: {exc}'
+    records.append(rec)
+    print(label, rec['name'], 'header' if rec['header'] else rec.get('header_error'))
+  with open(os.path.join(HERE, 'kernel_names.json'), 'w', encoding='utf-8') as f:
+    json.dump(records, f, indent=1)
+    f.write('\n')
+
+
+if __name__ == '__main__':
+  main()
