@@ -9,11 +9,14 @@ names and prototypes it produces):
 
 The kernel name is ``kernel_`` + the first 10 hex digits of the md5 of the description string
 (reference ``tensorforge/generators/generator.py:349-362``); the header is the launcher prototype
-(``generator.py:426-436``). The reference is asked for an arch from its own table (``gfx90a``);
-neither the name nor the prototype depends on the arch.
+(``generator.py:426-436``); the launch shape is the block (lanes per element, elements per block)
+and the dynamic LDS size in words that the reference's launcher passes
+(``generator.py:438-492``, chosen by ``RegmaxBlockPolicy`` and the LDS allocator), for the
+reference's own ``gfx950`` entry (``common/vm/hw_descr_db.yml``).
 """
 import json
 import os
+import re
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -32,13 +35,18 @@ def main():
   fp = {'double': FloatingPointType.DOUBLE, 'float': FloatingPointType.FLOAT}
   records = []
   for label, fp_name, descrs in name_cases.cases():
-    ctx = Context(arch='gfx90a', backend='hip', fp_type=fp[fp_name])
+    ctx = Context(arch='gfx950', backend='hip', fp_type=fp[fp_name])
     gen = Generator(descrs, ctx)
     gen.register()
     rec = {'label': label, 'fp_type': fp_name, 'name': gen.get_base_name(), 'header': None}
     try:
       gen.generate()
       rec['header'] = gen.get_header()
+      launcher = gen.get_launcher()
+      block = re.search(r'dim3 block\s*\(([^)]*)\)', launcher)
+      words = re.search(r'(\d+) \* sizeof\(', launcher)
+      rec['block'] = [int(v) for v in block.group(1).split(',')] if block else None
+      rec['shared_words'] = int(words.group(1)) if words else 0
     except Exception as exc:  # the name is still pinned; say why the prototype is not
       rec['header_error'] = f'{type(exc).__name__}: {exc}'
     records.append(rec)

@@ -58,6 +58,28 @@ def test_name_and_prototype_match_the_reference(case):
   assert ' '.join(gen.get_header().split()) == ' '.join(gold['header'].split())
 
 
+@pytest.mark.parametrize('case', name_cases.cases(), ids=lambda c: c[0])
+def test_plain_launch_shape_matches_the_reference(case):
+  """The plain emitter restates the reference's thread mapping: the same block (lanes per
+  element, elements per block) as the reference's launcher for gfx950. Its LDS layout is its
+  own (no per-element alignment padding, only the operands it stages), never larger than the
+  reference's, so its occupancy is never lower."""
+  from tensorforge.common.basic_types import FloatingPointType
+  from tensorforge.common.context import Context
+  from tensorforge.generators.generator import Generator
+  label, fp_name, descrs = case
+  gold = GOLDEN[label]
+  if gold.get('block') is None:
+    pytest.skip('the reference generates no launcher for this case')
+  fp = {'double': FloatingPointType.DOUBLE, 'float': FloatingPointType.FLOAT}[fp_name]
+  gen = Generator(descrs, Context(arch='gfx950', backend='hip', fp_type=fp), emitter='plain')
+  gen.generate()
+  em = gen.get_emitted()
+  assert list(em.block) == gold['block']
+  words = em.schedule['lds_words_per_element']
+  assert words * em.elements_per_block <= gold['shared_words']
+
+
 def test_catalog_header_exports_the_reference_names():
   """The prebuilt ADER and GEMM launchers are declared under the reference's names."""
   with open(os.path.join(REPO, 'include', 'tensorforge_catalog.h'), encoding='utf-8') as f:
