@@ -34,8 +34,8 @@ def parse():
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--cpu-seconds', type=float, default=25.0)
   ap.add_argument('--only', default='',
-                  help='comma list of legs (tuned,plain,f32,copy,mixed,precision,flux,tuner,'
-                       'pointwise,gemm)')
+                  help='comma list of legs (tuned,plain,f32,copy,mixed,precision,flux,deriv,'
+                       'tuner,pointwise,gemm)')
   return ap.parse_args()
 
 
@@ -272,6 +272,45 @@ def flux_leg(device, steps, warmup, n=262144):
   return res
 
 
+def deriv_leg(device, steps, warmup, n=262144):
+  """Time-derivative step (reference benchmarks/fused/programs/seissol-deriv_align.cf and
+  seissol-deriv_no_align.cf), FP64, on the same inputs: the default emitter choice (streaming
+  when aligned; plain when not aligned, where the column-per-lane form holds one wavefront per
+  CU) against the plain launcher, with the HBM roofline of the algorithmic bytes."""
+  from tensorforge_amd import catalog
+  from tensorforge_amd import workloads as W
+  from tensorforge_amd.common.context import Context
+  from tensorforge_amd.generators.generator import Generator
+  from tensorforge_amd.tune.measure import synthetic_buffers
+  res = {}
+  for label, stem in (('aligned', 'deriv'), ('not_aligned', 'deriv_na')):
+    entry, tuned = catalog.load(f'{stem}_f64_tuned')
+    _, plain = catalog.load(f'{stem}_f64_plain')
+    fresh = W.ader_derivative(entry.fp_type, aligned=(label == 'aligned'))
+    default = Generator(fresh.descrs, Context('gfx950', 'hip', entry.fp_type))
+    default.generate()
+    choice = default.get_emitted().emitter
+    bufs = synthetic_buffers(tuned.generator, n, device)
+    bufs.pop('__keep__')
+    em = tuned.generator.get_emitted()
+    bpe = em.bytes_per_element
+    t_t = time_kernel(tuned, bufs, n, steps if em.schedule.get('kind') == 'stream' else 3, 2)
+    t_p = time_kernel(plain, bufs, n, max(3, steps // 5), 2)
+    t_d = t_t if choice == 'tuned' else t_p
+    form = 'streaming' if em.schedule.get('kind') == 'stream' else 'column-per-lane'
+    res[label] = {'default': {'emitter': choice if choice == 'plain' else form,
+                              'value': n / t_d, 'ms_per_step': t_d * 1e3,
+                              'hbm_GBps': bpe * n / t_d / 1e9,
+                              'frac_of_peak': bpe * n / t_d / 1e9 / HBM_PEAK_GBPS},
+                  'tuned': {'form': form, 'ms_per_step': t_t * 1e3},
+                  'plain': {'value': n / t_p, 'ms_per_step': t_p * 1e3},
+                  'default_over_plain': t_p / t_d, 'bytes_per_element': bpe}
+    del bufs
+  res['unit'] = 'elements/s'
+  res['elements'] = n
+  return res
+
+
 def gemm_leg(device, steps, warmup, cpu_seconds, with_cpu, n=65536):
   """Single batched GEMM C_e += A * B_e (A 56x56 shared, B_e and C_e 56x9, seed 0, FP64,
   65,536 elements): tuned and plain launchers with the HBM roofline (12,096 B per element), the
@@ -483,7 +522,8 @@ def main():
   from tensorforge_amd import catalog
   device = torch.device('cuda:0')
   legs = set(args.only.split(',')) if args.only else {'tuned', 'plain', 'f32', 'copy', 'mixed',
-                                                      'precision', 'flux', 'tuner', 'pointwise',
+                                                      'precision', 'flux', 'deriv', 'tuner',
+                                                      'pointwise',
                                                       'gemm'}
   n = args.elements
   out = {}
@@ -548,6 +588,8 @@ def main():
 
   if 'flux' in legs:
     out['flux'] = flux_leg(device, args.steps, args.warmup)
+  if 'deriv' in legs:
+    out['derivative'] = deriv_leg(device, args.steps, args.warmup)
 
   if 'pointwise' in legs:
     out['pointwise'] = pointwise_leg(device, args.steps, args.warmup)

@@ -46,8 +46,17 @@ _REGISTRY: Dict[str, Callable[[], Entry]] = {}
 
 
 def register(name):
+  """Register an entry factory; an entry named ``*_plain`` is always built with the plain
+  emitter (the baseline its ``*_tuned`` / ``*_auto`` siblings are compared with)."""
   def deco(fn):
-    _REGISTRY[name] = fn
+    if name.endswith('_plain'):
+      def make(fn=fn):
+        entry = fn()
+        entry.emitter = 'plain'
+        return entry
+      _REGISTRY[name] = make
+    else:
+      _REGISTRY[name] = fn
     return fn
   return deco
 
@@ -190,6 +199,15 @@ for _emitter in ('plain', 'tuned'):
       lambda e=_emitter, o=_ord: _wl_entry(W.flux_local(F64, o), e))
     register(f'flux_neigh_{_ord}_f64_{_emitter}')(
       lambda e=_emitter, o=_ord: _wl_entry(W.flux_neigh(F64, o), e))
+
+# time-derivative step (reference benchmarks/fused/programs/seissol-deriv_*.cf): aligned (one
+# streamed array, per-term contraction ranges 53/54/55) and not aligned (three streamed arrays
+# from row 1, 35 destination rows)
+for _emitter in ('plain', 'tuned'):
+  register(f'deriv_f64_{_emitter}')(
+    lambda e=_emitter: _wl_entry(W.ader_derivative(F64, aligned=True), e))
+  register(f'deriv_na_f64_{_emitter}')(
+    lambda e=_emitter: _wl_entry(W.ader_derivative(F64, aligned=False), e))
 
 
 # precision study: order 7 (84 basis functions), dense K_d, FP64 inputs and outputs

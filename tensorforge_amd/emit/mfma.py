@@ -42,6 +42,8 @@ def unsupported_reason(program, pl):
     return 'a term without a per-element temporary'
   if any(t.U.is_sparse() for t in pl.terms):
     return 'sparse shared operand'
+  if any(tuple(t.product.k_ranges[0]) != (pl.klo, pl.khi) for t in pl.terms):
+    return 'terms with different contraction ranges'
   if pl.L is None or pl.L.arg.addressing != Addressing.STRIDED:
     return 'streamed operand is not strided'
   if pl.l_cols[1] - pl.l_cols[0] != pl.q or pl.q > 16:

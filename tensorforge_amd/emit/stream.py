@@ -209,15 +209,20 @@ def _plan_single(program):
     U = sh.left
     if U.src != ir.SRC_GLOBAL or U.arg.addressing != Addressing.NONE:
       return None, 'left operand is not shared by all elements'
-    if tuple(p.k_ranges[0]) != (klo, khi) and klo is not None:
-      return None, 'products with different contraction ranges'
-    klo, khi = p.k_ranges[0]
+    # terms whose temporaries stream one operand may contract over different leading ranges
+    # [0, k_t) of it (each term's entries stay in its own range); the plan's range is the widest
+    kr = tuple(p.k_ranges[0])
     R = sh.right
+    if klo is not None and kr != (klo, khi) and \
+        (R.src != ir.SRC_TMP or kr[0] != 0 or klo != 0 or
+         any(t.kind != 'fused' for t in terms)):
+      return None, 'products with different contraction ranges'
+    klo, khi = kr[0], max(kr[1], khi or 0)
     if R.src == ir.SRC_TMP:
       if id(R.tensor) not in by_tmp or sh.right_dims != (0, 1):
         return None, 'temporary not formed by a streamable product'
       prod, psh = by_tmp.pop(id(R.tensor))
-      if tuple(prod.n_ranges[0]) != (0, khi) or klo != 0:
+      if prod.n_ranges[0][0] != 0 or prod.n_ranges[0][1] < kr[1] or kr[0] != 0:
         return None, 'temporary rows do not match the contraction'
       if prod.n_ranges[1][0] != 0 or prod.n_ranges[1][1] < q:
         return None, 'temporary columns do not match'
@@ -229,6 +234,8 @@ def _plan_single(program):
       terms.append(Term(p, U, sh.left_dims, 'fused', producer=prod, W=psh.right,
                         w_dims=psh.right_dims))
     elif R.src == ir.SRC_GLOBAL and R.arg.per_element and not R.is_sparse():
+      if any(tuple(t.product.k_ranges[0]) != kr for t in terms):
+        return None, 'products with different contraction ranges'
       terms.append(Term(p, U, sh.left_dims, 'global', B=R, b_dims=sh.right_dims))
     else:
       return None, 'right operand kind'
@@ -280,9 +287,10 @@ class _Stream:
       else:
         items = [(idx, U.offset(idx)) for idx in
                  ir.iterate_box([(U.origin[d], U.origin[d] + U.tensor.shape[d]) for d in range(2)])]
+      tk0, tk1 = t.product.k_ranges[0]
       for idx, pos in items:
         i, k = idx[nd], idx[kd]
-        if pl.rows[0] <= i < pl.rows[1] and pl.klo <= k < pl.khi:
+        if pl.rows[0] <= i < pl.rows[1] and max(pl.klo, tk0) <= k < min(pl.khi, tk1):
           t.entries.append((i - lo0, k, pos))
     blocks = list(range(pl.klo, pl.khi, self.KB))
     for b0 in blocks:
@@ -320,6 +328,8 @@ class _Stream:
     if self.sched.stream_loop == 'never' or pl.post:
       return False
     if any(t.U.is_sparse() for t in pl.terms):
+      return False
+    if any(tuple(t.product.k_ranges[0]) != (pl.klo, pl.khi) for t in pl.terms):
       return False
     if (pl.khi - pl.klo) % max(2, self.sched.stream_loop_k_block) != 0:
       return False

@@ -837,6 +837,16 @@ class _Emitter:
           w(f'g_{key}[{self._dest_off(p, lo0 + i, lo1)}] = {d["name"]}[{i}];')
 
 
+def starved(emitted) -> bool:
+  """A column-per-lane kernel whose LDS per workgroup leaves fewer than four wavefronts
+  resident on a CU (one per SIMD): nothing hides its latencies, and the plain kernel, with
+  less LDS per element, runs faster."""
+  if emitted.schedule.get('kind') == 'stream' or not emitted.lds_bytes:
+    return False
+  waves = emitted.block[0] * emitted.block[1] * emitted.block[2] // 64
+  return (LDS_LIMIT // emitted.lds_bytes) * waves < 4
+
+
 def emit(program, context, base_name, schedule=None) -> Emitted:
   from . import stream
   schedule = schedule or Schedule()

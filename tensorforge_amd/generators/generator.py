@@ -186,6 +186,9 @@ class Generator:
         except GenerationError:
           if choice == 'tuned':
             raise
+        if choice == 'auto' and emitted is not None and tuned.starved(emitted) and \
+            self._program.epilogue is None:
+          emitted = None      # the plain kernel keeps more wavefronts resident
       elif choice == 'tuned':
         raise GenerationError(f'tuned emitter cannot generate this chain: {reason}')
     if emitted is None:

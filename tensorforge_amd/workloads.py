@@ -170,6 +170,40 @@ def flux_neigh(fp_type=FloatingPointType.DOUBLE, ordering='mod') -> Workload:
   return Workload('flux_neigh', descrs, fp_type, tensors, {'ordering': ordering, 'b': 56, 'q': 9})
 
 
+def ader_derivative(fp_type=FloatingPointType.DOUBLE, aligned=True) -> Workload:
+  """One step of the ADER time-derivative recursion (reference
+  ``benchmarks/fused/programs/seissol-deriv_align.cf`` and ``seissol-deriv_no_align.cf``):
+  ``D += C·(A·B) + F·(A'·E) + H·(A''·G)`` with rectangular shared matrices (48×53, 48×54,
+  48×55) contracting over the first 53/54/55 rows of 64-row per-element arrays, 9×9 per-element
+  star matrices. Aligned: one streamed array ``A`` read from row 0, all 48 rows of ``D``.
+  Not aligned: three streamed arrays read from row 1 (the previous derivative's constant mode
+  dropped), 35 of 48 rows of ``D`` and of the shared matrices."""
+  BB = BoundingBox
+  sub = lambda t: SubTensor(t, t.bbox)
+  pair, ident = [[0, -1], [-1, 1]], [[0, 1], [0, 1]]
+  rows = 48 if aligned else 35
+  D = Tensor([48, 9], Addressing.STRIDED, BB([0, 0], [rows, 9]), alias='D')
+  tensors = {'D': D}
+  descrs = []
+  for d, kcols in enumerate((53, 54, 55)):
+    if aligned:
+      if d == 0:
+        tensors['A'] = Tensor([64, 9], Addressing.STRIDED, BB([0, 0], [64, 9]), alias='A')
+      A = tensors['A']
+    else:
+      A = Tensor([64, 9], Addressing.STRIDED, BB([1, 0], [1 + kcols, 9]), alias=f'A{d + 1}')
+      tensors[f'A{d + 1}'] = A
+    S = Tensor([9, 9], Addressing.STRIDED, BB([0, 0], [9, 9]), alias=f'S{d + 1}')
+    K = Tensor([48, kcols], Addressing.NONE, BB([0, 0], [rows, kcols]), alias=f'K{d + 1}')
+    tensors[f'S{d + 1}'] = S
+    tensors[f'K{d + 1}'] = K
+    T = generate_tmp_tensor([sub(A), sub(S)], pair)
+    descrs.append(MultilinearDescr(SubTensor(T), [sub(A), sub(S)], pair, ident))
+    descrs.append(MultilinearDescr(sub(D), [sub(K), SubTensor(T)], pair, ident, add=True))
+  return Workload('ader_derivative', descrs, fp_type, tensors,
+                  {'aligned': aligned, 'rows': rows, 'q': 9})
+
+
 # uneven batching: orders 2..7 (basis sizes) and their share of the elements
 MIXED_BASES = (4, 10, 20, 35, 56, 84)
 MIXED_WEIGHTS = (0.05, 0.1, 0.2, 0.3, 0.25, 0.1)

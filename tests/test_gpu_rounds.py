@@ -10,6 +10,7 @@ oracle). Normwise per element: 1e-12 FP64, 1e-5 FP32 and FP32 MFMA, 1e-4 split B
 import pytest
 
 from tensorforge_amd import catalog
+from tensorforge_amd.common.basic_types import FloatingPointType
 
 from harness import gpu_run, key_of, make_inputs, normwise_error, outputs_of
 
@@ -20,7 +21,8 @@ PAIRS = [(t, t.rsplit('_', 1)[0] + '_plain', 1e-5 if '_f32_' in t else 1e-12)
          for t in catalog.names() if t.endswith('_tuned') and not t.startswith('t_')]
 PAIRS += [('ader7_f64_mfma_fp32', 'ader7_f64_plain', 1e-5),
           ('ader7_f64_mfma_bf16x3', 'ader7_f64_plain', 1e-4)]
-PAIRS += [(t, t[:-len('_auto')] + '_plain', 1e-5 if 'f32' in t else 1e-12)
+PAIRS += [(t, t[:-len('_auto')] + '_plain',
+           1e-5 if catalog.get(t).fp_type == FloatingPointType.FLOAT else 1e-12)
           for t in catalog.names() if t.startswith('t_') and t.endswith('_auto')]
 # the streaming emitter's schedule options (catalog opt_<chain>_<option>)
 PAIRS += [(t, t[len('opt_'):t.rindex('_')] + '_plain', 1e-12)
