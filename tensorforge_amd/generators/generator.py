@@ -268,6 +268,9 @@ class Generator:
     return f'void launcher_{self._base_kernel_name}({", ".join(params)})'
 
   def default_generate_call_site(self):
+    """The reference's default call-site string (``generator.py:445-459``): global operands
+    under their aliases, then ``flags, streamPtr`` appended to a parameter list that already
+    ends in ``numElements, flags`` (so ``flags`` appears twice, as there)."""
     if not self._is_registerd:
       raise RuntimeError('generator is not registered. Call register first.')
     symbols = deepcopy(self._global_symbols())

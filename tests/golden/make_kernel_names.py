@@ -9,7 +9,9 @@ names and prototypes it produces):
 
 The kernel name is ``kernel_`` + the first 10 hex digits of the md5 of the description string
 (reference ``tensorforge/generators/generator.py:349-362``); the header is the launcher prototype
-(``generator.py:426-436``); the launch shape is the block (lanes per element, elements per block)
+(``generator.py:426-436``); the call sites are ``default_generate_call_site()`` and
+``generate_call_site`` with ``<alias>_ptr`` / ``<alias>_off`` names (``generator.py:445-492``);
+the launch shape is the block (lanes per element, elements per block)
 and the dynamic LDS size in words that the reference's launcher passes
 (``generator.py:438-492``, chosen by ``RegmaxBlockPolicy`` and the LDS allocator), for the
 reference's own ``gfx950`` entry (``common/vm/hw_descr_db.yml``).
@@ -38,7 +40,13 @@ def main():
     ctx = Context(arch='gfx950', backend='hip', fp_type=fp[fp_name])
     gen = Generator(descrs, ctx)
     gen.register()
-    rec = {'label': label, 'fp_type': fp_name, 'name': gen.get_base_name(), 'header': None}
+    rec = {'label': label, 'fp_type': fp_name, 'name': gen.get_base_name(), 'header': None,
+           'default_call_site': gen.default_generate_call_site()}
+    mats, offs = name_cases.call_site_maps(descrs)
+    try:
+      rec['call_site'] = gen.generate_call_site(mats, offs, 'numElements', 'flags', 'stream')
+    except Exception as exc:
+      rec['call_site_error'] = f'{type(exc).__name__}: {exc}'
     try:
       gen.generate()
       rec['header'] = gen.get_header()

@@ -137,3 +137,20 @@ def cases():
               [MultilinearDescr(SubTensor(D3), [SubTensor(A3), SubTensor(B3)],
                                 [[0, -1], [-1, 1, 2]], [[0, 1], [0, 1, 2]])]))
   return out
+
+
+def aliases(descrs):
+  """Aliases of the named tensors of a description list, in first-appearance order."""
+  out = []
+  for d in descrs:
+    for sub in [d.dest] + list(d.ops):
+      alias = getattr(getattr(sub, 'tensor', sub), 'alias', None)
+      if alias is not None and alias not in out:
+        out.append(alias)
+  return out
+
+
+def call_site_maps(descrs):
+  """Name maps for ``Generator.generate_call_site``: ``<alias>_ptr`` / ``<alias>_off``."""
+  names = aliases(descrs)
+  return ({a: f'{a}_ptr' for a in names}, {a: f'{a}_off' for a in names})

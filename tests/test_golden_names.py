@@ -2,8 +2,9 @@
 
 ``tests/golden/kernel_names.json`` holds, for every case of ``tests/golden/name_cases.py``, the
 name (``kernel_`` + md5 prefix of the description string, reference
-``tensorforge/generators/generator.py:349-362``) and the launcher header
-(``generator.py:426-436``) that the reference generator produced for it
+``tensorforge/generators/generator.py:349-362``), the launcher header (``generator.py:426-436``),
+the call-site strings (``generator.py:445-492``) and the launch block that the reference
+generator produced for it
 (``tests/golden/make_kernel_names.py``). The same cases are built here with this package; a
 caller that links launchers by name and declares them with the reference's header finds the same
 symbols with the same parameters. Pointwise (``ElementwiseDescr``) names are not among them: the
@@ -78,6 +79,18 @@ def test_plain_launch_shape_matches_the_reference(case):
   assert list(em.block) == gold['block']
   words = em.schedule['lds_words_per_element']
   assert words * em.elements_per_block <= gold['shared_words']
+
+
+@pytest.mark.parametrize('case', name_cases.cases(), ids=lambda c: c[0])
+def test_call_sites_match_the_reference(case):
+  label, fp_name, descrs = case
+  gold = GOLDEN[label]
+  gen = _generate(fp_name, descrs)
+  norm = lambda text: ' '.join(text.split())
+  assert norm(gen.default_generate_call_site()) == norm(gold['default_call_site'])
+  mats, offs = name_cases.call_site_maps(descrs)
+  assert norm(gen.generate_call_site(mats, offs, 'numElements', 'flags', 'stream')) == \
+      norm(gold['call_site'])
 
 
 def test_catalog_header_exports_the_reference_names():
