@@ -82,6 +82,10 @@ def emit(program: ir.ChainProgram, context, base_name, num_threads, policy_type)
     plan['lds'] = offset
     lds_per_elem = max(lds_per_elem, offset)
     plans.append(plan)
+  if lds_per_elem > 0:
+    # each element's region padded as the reference sizes it: aligned to the vector unit's width
+    # in words, plus its bank term (reference common/context.py:38-44, backend/opt/optimizer.py:35-37)
+    lds_per_elem = context.align(lds_per_elem) + (lanes % 32) // 4 * 4
 
   policy = policy_type(context, 0, lds_per_elem, lanes)
   mults = policy.get_num_mults_per_block()
