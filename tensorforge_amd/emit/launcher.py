@@ -141,7 +141,10 @@ def _launch_block(w, generator, emitted):
     with w.block('static bool shmemsizeset = false; if (!shmemsizeset)'):
       w(f'hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&{kname}), '
         f'hipFuncAttributeMaxDynamicSharedMemorySize, {emitted.lds_bytes});')
-      w(f'if (err != hipSuccess) {{ tensorforge_record_error(static_cast<int>(err), '
+      # the failed call leaves HIP's last error set: clear it, or the next launcher that
+      # succeeds would read it back as its own failure
+      w(f'if (err != hipSuccess) {{ (void)hipGetLastError(); '
+        f'tensorforge_record_error(static_cast<int>(err), '
         f'"launcher_{generator.get_base_name()}"); return; }}')
       w('shmemsizeset = true;')
   args = ', '.join(kernel_args(program))
@@ -149,6 +152,63 @@ def _launch_block(w, generator, emitted):
   w('hipError_t err = hipGetLastError();')
   w(f'if (err != hipSuccess) tensorforge_record_error(static_cast<int>(err), '
     f'"launcher_{generator.get_base_name()}");')
+
+
+def persistent_grid(kname: str, lds_bytes: int, threads: int, per_block: int, where: str,
+                    groups_expr: str = '') -> str:
+  """C statements sizing the grid of a persistent kernel: as many workgroups as stay resident
+  (CUs x workgroups per CU, queried once), at most one per ``per_block`` elements. Every HIP
+  call's status is checked: on a failure (or a kernel that cannot be resident at all) the error
+  is recorded, HIP's last error is cleared so later launchers do not read it back, and the
+  launcher returns without launching. ``groups_expr`` overrides the number of workgroup-sized
+  element groups."""
+  return (f'static int tf_resident = 0;\n'
+          f'if (tf_resident == 0) {{\n'
+          f'  int dev = 0, cus = 0, per_cu = 0;\n'
+          f'  hipError_t qerr = hipGetDevice(&dev);\n'
+          f'  if (qerr == hipSuccess) qerr = hipDeviceGetAttribute(&cus, '
+          f'hipDeviceAttributeMultiprocessorCount, dev);\n'
+          f'  if (qerr == hipSuccess) qerr = hipFuncSetAttribute(reinterpret_cast<const void*>('
+          f'&{kname}), hipFuncAttributeMaxDynamicSharedMemorySize, {lds_bytes});\n'
+          f'  if (qerr == hipSuccess) qerr = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, '
+          f'{kname}, {threads}, {lds_bytes});\n'
+          f'  if (qerr == hipSuccess && (cus <= 0 || per_cu <= 0)) qerr = hipErrorInvalidConfiguration;\n'
+          f'  if (qerr != hipSuccess) {{\n'
+          f'    (void)hipGetLastError();\n'
+          f'    tensorforge_record_error(static_cast<int>(qerr), "{where}");\n'
+          f'    return;\n'
+          f'  }}\n'
+          f'  tf_resident = cus * per_cu;\n'
+          f'}}\n'
+          + (f'const size_t tf_groups = {groups_expr};\n' if groups_expr else
+             f'const size_t tf_groups = (numElements + {per_block - 1}) / {per_block};\n') +
+          f'const unsigned tf_grid = static_cast<unsigned>(tf_groups < (size_t)tf_resident ? '
+          f'tf_groups : (size_t)tf_resident);')
+
+
+def distinct_symbol(emitted: Emitted) -> Emitted:
+  """Rename the ``__global__`` symbol of a non-plain kernel to
+  ``<name>_<form>_<digest of its source>``. The launcher keeps the reference's name; the plain
+  kernel keeps the reference's kernel name. Different forms (and schedules) of one chain then
+  export different kernel symbols, so two objects holding forms of one chain can be loaded into
+  one process without their kernels or their launchers' one-time statics being confused, and a
+  profiler's trace names each form."""
+  import dataclasses
+  import hashlib
+  if emitted.emitter == 'plain':
+    return emitted
+  form = str(emitted.schedule.get('kind', emitted.emitter))
+  digest = hashlib.sha1(emitted.full_source().encode()).hexdigest()[:8]
+  old = emitted.kernel_name
+  new = f'{old}_{form}_{digest}'
+
+  def rename(em):
+    fb = rename(em.fallback) if em.fallback is not None else None
+    return dataclasses.replace(em, kernel_name=em.kernel_name.replace(old, new),
+                               kernel_src=em.kernel_src.replace(old, new),
+                               extra_launch_setup=em.extra_launch_setup.replace(old, new),
+                               condition=em.condition.replace(old, new), fallback=fb)
+  return rename(emitted)
 
 
 def translation_unit(generators) -> str:

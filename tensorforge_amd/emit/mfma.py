@@ -25,7 +25,7 @@ sum.
 """
 from ..common.basic_types import Addressing, FloatingPointType
 from . import ir
-from .launcher import Emitted, Writer, kernel_params
+from .launcher import Emitted, Writer, kernel_params, persistent_grid
 from .traffic import element_traffic
 
 WAVE = 64
@@ -52,12 +52,25 @@ def unsupported_reason(program, pl):
     return 'more than 128 rows'
   if pl.init == 'load' and pl.beta != 1.0:
     return 'scaled accumulation'
+  # register stages after the streamed product (the flux integrals' U_n ... U_2 chain) and
+  # their final accumulation are not part of this form
+  if pl.post or pl.final_init != 'zero':
+    return 'register stages after the streamed product'
+  # every per-element operand (the destination and the weights included) is addressed as
+  # element * volume + extraOffset
+  if pl.dest.addressing != Addressing.STRIDED:
+    return 'destination is not strided'
+  if any(a.per_element and a.addressing != Addressing.STRIDED for a in program.args):
+    return 'a per-element operand is not strided'
+  if program.epilogue is not None:
+    return 'pointwise statements fused into the chain'
   return None
 
 
 class _Mfma:
   def __init__(self, program, context, base_name, pl, mode):
     self.prog, self.ctx, self.pl, self.mode = program, context, pl, mode
+    self.base_name = base_name
     self.kname = f'kernel_{base_name}'
     self.w = Writer()
     self.q = pl.q
@@ -402,21 +415,8 @@ class _Mfma:
           w('dst[off] = v;')
 
   def grid_setup(self, lds_bytes, threads):
-    G = self.G
-    return (f'static int tf_resident = 0;\n'
-            f'if (tf_resident == 0) {{\n'
-            f'  int dev = 0, cus = 0, per_cu = 0;\n'
-            f'  (void)hipGetDevice(&dev);\n'
-            f'  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);\n'
-            f'  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&{self.kname}), '
-            f'hipFuncAttributeMaxDynamicSharedMemorySize, {lds_bytes});\n'
-            f'  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, {self.kname}, {threads}, '
-            f'{lds_bytes});\n'
-            f'  tf_resident = (cus > 0 ? cus : 256) * (per_cu > 0 ? per_cu : 1);\n'
-            f'}}\n'
-            f'const size_t tf_groups = (numElements + {G * WAVES - 1}) / {G * WAVES};\n'
-            f'const unsigned tf_grid = static_cast<unsigned>(tf_groups < (size_t)tf_resident ? '
-            f'tf_groups : (size_t)tf_resident);')
+    return persistent_grid(self.kname, lds_bytes, threads, self.G * WAVES,
+                           f'launcher_{self.base_name}')
 
 
 def emit(program, context, base_name, mode):

@@ -25,7 +25,7 @@ from typing import List, Optional
 from ..common.basic_types import Addressing, FloatingPointType
 from ..common.exceptions import GenerationError
 from . import ir
-from .launcher import Emitted, Writer, kernel_params
+from .launcher import Emitted, Writer, kernel_params, persistent_grid
 from .traffic import element_traffic
 
 WAVE = 64
@@ -256,6 +256,7 @@ class _Stream:
     self.real = FloatingPointType.as_str(self.fp)
     self.sched = schedule
     self.pl = pl
+    self.base_name = base_name
     self.kname = f'kernel_{base_name}{suffix}'
     self.w = Writer()
     self.q = pl.q
@@ -497,21 +498,8 @@ class _Stream:
                    condition=self.dma_condition() if self.dma else '')
 
   def grid_setup(self, lds_bytes, threads):
-    G, W = self.G, self.W
-    return (f'static int tf_resident = 0;\n'
-            f'if (tf_resident == 0) {{\n'
-            f'  int dev = 0, cus = 0, per_cu = 0;\n'
-            f'  (void)hipGetDevice(&dev);\n'
-            f'  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);\n'
-            f'  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&{self.kname}), '
-            f'hipFuncAttributeMaxDynamicSharedMemorySize, {lds_bytes});\n'
-            f'  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, {self.kname}, {threads}, '
-            f'{lds_bytes});\n'
-            f'  tf_resident = (cus > 0 ? cus : 256) * (per_cu > 0 ? per_cu : 1);\n'
-            f'}}\n'
-            f'const size_t tf_groups = (numElements + {G * W - 1}) / {G * W};\n'
-            f'const unsigned tf_grid = static_cast<unsigned>(tf_groups < (size_t)tf_resident ? '
-            f'tf_groups : (size_t)tf_resident);')
+    return persistent_grid(self.kname, lds_bytes, threads, self.G * self.W,
+                           f'launcher_{self.base_name}')
 
   def _select(self, values):
     if all(v == values[0] for v in values):

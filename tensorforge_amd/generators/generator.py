@@ -154,7 +154,8 @@ class Generator:
       if self._emitter_choice not in ('auto', 'tuned', 'plain'):
         raise GenerationError(f'{self._emitter_choice} does not generate pointwise kernels')
       self._program = pw.lower(self.descr_list, self._global_tensors, self._context.fp_type)
-      emitted = pw.emit(self._program, self._context, self._base_kernel_name)
+      emitted = launcher.distinct_symbol(pw.emit(self._program, self._context,
+                                                 self._base_kernel_name))
       self._emitted = emitted
       self._kernel = emitted.full_source()
       self._launcher = launcher.launcher_source(self, emitted)
@@ -196,6 +197,7 @@ class Generator:
         raise ValueError(f'unknown emitter {choice!r}')
       emitted = plain.emit(self._program, self._context, self._base_kernel_name,
                            self._num_threads, self._thread_block_policy_type)
+    emitted = launcher.distinct_symbol(emitted)
     self._emitted = emitted
     self._kernel = emitted.full_source()
     self._launcher = launcher.launcher_source(self, emitted)

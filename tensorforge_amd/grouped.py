@@ -19,7 +19,7 @@ import hashlib
 
 from .common.basic_types import GeneralLexicon
 from .common.exceptions import GenerationError
-from .emit.launcher import HEADER, Writer, kernel_args, kernel_params
+from .emit.launcher import HEADER, Writer, kernel_args, kernel_params, persistent_grid
 from .emit.tuned import LDS_LIMIT
 
 
@@ -145,22 +145,9 @@ class GroupedGenerator:
         f'static_cast<hipStream_t>({GeneralLexicon.STREAM_PTR_STR}) : 0;')
       groups = ' + '.join(f'(g{i}_{GeneralLexicon.NUM_ELEMENTS} + {pb - 1}) / {pb}'
                           for i, pb in enumerate(self.per_block))
-      w(f'const size_t tf_groups = {groups};')
-      w('if (tf_groups == 0) return;')
-      w('static int tf_resident = 0;')
-      with w.block('if (tf_resident == 0)'):
-        w('int dev = 0, cus = 0, per_cu = 0;')
-        w('(void)hipGetDevice(&dev);')
-        w('(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);')
-        w(f'hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(&{kname}), '
-          f'hipFuncAttributeMaxDynamicSharedMemorySize, {self.lds_bytes});')
-        w(f'if (err != hipSuccess) {{ tensorforge_record_error(static_cast<int>(err), '
-          f'"launcher_{self._name}"); return; }}')
-        w(f'(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, {kname}, {threads}, '
-          f'{self.lds_bytes});')
-        w('tf_resident = (cus > 0 ? cus : 256) * (per_cu > 0 ? per_cu : 1);')
-      w('const unsigned tf_grid = static_cast<unsigned>(tf_groups < (size_t)tf_resident ? '
-        'tf_groups : (size_t)tf_resident);')
+      w('if ((' + groups + ') == 0) return;')
+      w.lines(persistent_grid(kname, self.lds_bytes, threads, 1, f'launcher_{self._name}',
+                              groups_expr=groups))
       args = []
       for i, g in enumerate(self.members):
         args += [f'g{i}_{a}' for a in kernel_args(g.get_program())]

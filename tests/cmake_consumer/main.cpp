@@ -34,7 +34,12 @@ int main() {
   hipMemcpy(dC, C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice);
 
   tf_execute(dA, dB, dC, dt, 0u, 0u, n, nullptr, nullptr);
-  if (tensorforge::synchDevice() != 0 || CHECK_ERR != 0) return 2;
+  // the C status API gives the code; the reference's shim (device-wide sync, then the check)
+  // reports it on stderr
+  const int status = tensorforge_synch_device(nullptr);
+  tensorforge::synchDevice();
+  CHECK_ERR;
+  if (status != 0) return 2;
   hipMemcpy(C.data(), dC, C.size() * sizeof(double), hipMemcpyDeviceToHost);
   hipFree(dA);
   hipFree(dB);

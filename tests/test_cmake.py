@@ -58,26 +58,59 @@ def test_other_backends_are_refused(tmp_path):
   assert r.returncode != 0 and 'Allowed: hip' in (r.stdout + r.stderr)
 
 
+RT_SYMBOLS = ('_ZN11tensorforge8checkErrERKNSt7__cxx1112basic_stringIcSt11char_traitsIcESaIcEEEi',
+              '_ZN11tensorforge11synchDeviceEPv')
+
+
+def test_runtime_exports_the_reference_shim():
+  """libtensorforge_rt.so exports tensorforge::checkErr(const std::string&, int) and
+  tensorforge::synchDevice(void*) out of line, with the signatures of the reference's
+  tensorforge_aux.h, next to the C status functions."""
+  from tensorforge_amd.runtime.build import runtime_path
+  if not os.path.exists(runtime_path()):
+    pytest.skip('runtime library not built')
+  r = subprocess.run(['nm', '-D', '--defined-only', runtime_path()], capture_output=True, text=True)
+  assert r.returncode == 0
+  for sym in RT_SYMBOLS + ('tensorforge_last_error', 'tensorforge_synch_device'):
+    assert sym in r.stdout, sym
+
+
 @pytest.mark.skipif(shutil.which('cmake') is None, reason='cmake is not installed')
-def test_reference_integration_project(tmp_path):
-  """The reference's own CMake integration check (tests/cmake_integration/CMakeLists.txt:1-11):
-  CMAKE_PREFIX_PATH from print_cmake_path(), find_package(Tensorforge REQUIRED), the include
-  directories, a C++ file including tensorforge_aux.h."""
+def test_prefix_path_integration(tmp_path):
+  """A project that finds the package the way the reference's integration check does
+  (CMAKE_PREFIX_PATH from ``print_cmake_path()``, then ``find_package(Tensorforge)`` in that
+  spelling) and calls the shim through ``tensorforge_aux.h``: it must configure, compile and
+  link ``CHECK_ERR`` and ``tensorforge::synchDevice`` against the runtime library."""
+  from tensorforge_amd.runtime.build import runtime_path
+  if not os.path.exists(runtime_path()):
+    pytest.skip('runtime library not built')
   (tmp_path / 'CMakeLists.txt').write_text(
-    'cmake_minimum_required(VERSION 3.5)\nproject(cmake_integration)\n'
-    'execute_process(COMMAND python3 -c "import tensorforge; tensorforge.print_cmake_path()"\n'
-    '                OUTPUT_VARIABLE TENSORFORGE_PATH)\n'
-    'set(CMAKE_PREFIX_PATH "${TENSORFORGE_PATH}" ${CMAKE_MODULE_PATH})\n'
+    'cmake_minimum_required(VERSION 3.16)\n'
+    'project(tf_prefix_check CXX)\n'
+    f'execute_process(COMMAND "{sys.executable}" -c "import tensorforge as t; t.print_cmake_path()"\n'
+    '                OUTPUT_VARIABLE TF_PREFIX OUTPUT_STRIP_TRAILING_WHITESPACE)\n'
+    'list(APPEND CMAKE_PREFIX_PATH "${TF_PREFIX}")\n'
     'find_package(Tensorforge REQUIRED)\n'
-    'add_executable(${CMAKE_PROJECT_NAME} ./main.cpp)\n'
-    'target_include_directories(${CMAKE_PROJECT_NAME} PRIVATE ${TensorForge_INCLUDE_DIRS})\n')
-  (tmp_path / 'main.cpp').write_text(
-    '#include <tensorforge_aux.h>\n#include <iostream>\n'
-    'int main() { std::cout << "hello tensorforge" << std::endl; return 0; }\n')
+    'add_executable(shim_check shim_check.cpp)\n'
+    'target_include_directories(shim_check PRIVATE ${TensorForge_INCLUDE_DIRS} /opt/rocm/include)\n'
+    'target_compile_definitions(shim_check PRIVATE __HIP_PLATFORM_AMD__)\n'
+    'target_link_libraries(shim_check PRIVATE TensorForge::TensorForge)\n')
+  (tmp_path / 'shim_check.cpp').write_text(
+    '// links the reference shim\'s declarations against libtensorforge_rt.so\n'
+    '#include <tensorforge_aux.h>\n'
+    '#include <cstdio>\n'
+    'int main(int argc, char**) {\n'
+    '  if (argc > 1) { tensorforge::synchDevice(); CHECK_ERR; }\n'
+    '  std::printf("%d\\n", tensorforge_last_error());\n'
+    '  return 0;\n'
+    '}\n')
   env = dict(os.environ, PYTHONPATH=ROOT)
   for cmd in (['cmake', '-S', str(tmp_path), '-B', str(tmp_path / 'b')],
               ['cmake', '--build', str(tmp_path / 'b')]):
     r = subprocess.run(cmd, capture_output=True, text=True, env=env)
     assert r.returncode == 0, (r.stdout + r.stderr)[-3000:]
-  r = subprocess.run([str(tmp_path / 'b' / 'cmake_integration')], capture_output=True, text=True)
-  assert r.returncode == 0 and 'hello tensorforge' in r.stdout
+  exe = tmp_path / 'b' / 'shim_check'
+  assert os.access(exe, os.X_OK)
+  r = subprocess.run(['nm', '-u', str(exe)], capture_output=True, text=True)
+  for sym in RT_SYMBOLS:
+    assert sym in r.stdout, sym

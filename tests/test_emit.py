@@ -105,3 +105,40 @@ def test_row_bands_of_an_unrolled_kernel_have_their_own_tables():
   assert len(names) == 2
   for n in names:
     assert f'{n}[e]' in src
+
+
+def test_matrix_core_forms_refuse_register_stage_chains():
+  """The flux integrals stream an inner product and run outer stages on registers, with a final
+  accumulation into the destination: the matrix-core forms do not implement those stages and
+  must refuse the chains instead of generating a kernel that drops them."""
+  import pytest
+  from tensorforge_amd.common.exceptions import GenerationError
+  for make in (W.flux_local, W.flux_neigh):
+    w = make()
+    for mode in ('mfma_fp32', 'mfma_bf16x3'):
+      g = Generator(w.descrs, Context('gfx950', 'hip', FloatingPointType.DOUBLE), emitter=mode)
+      with pytest.raises(GenerationError, match='register stages'):
+        g.generate()
+
+
+def test_forms_of_one_chain_export_distinct_kernel_symbols():
+  """The launcher keeps the reference's name for every form; the plain kernel keeps the
+  reference's kernel name; other forms get a kernel symbol of their own (form and source
+  digest), so forms of one chain can share a process and a trace."""
+  w = W.ader_volume(b=84, sparse_k=False, star=True)
+  names = {}
+  for emitter in ('plain', 'tuned', 'mfma_fp32', 'mfma_bf16x3'):
+    g = Generator(w.descrs, Context('gfx950', 'hip', FloatingPointType.DOUBLE), emitter=emitter)
+    g.generate()
+    em = g.get_emitted()
+    names[emitter] = em.kernel_name
+    assert f'launcher_{g.get_base_name()}(' in g.get_launcher()
+    assert re.search(rf'__global__ void __launch_bounds__\([^)]*\) {em.kernel_name}\(',
+                     g.get_kernel())
+    assert f'&{em.kernel_name})' in g.get_launcher() or f'({em.kernel_name},' in g.get_launcher()
+  base = 'kernel_' + g.get_base_name()
+  assert names['plain'] == base
+  assert names['tuned'].startswith(base + '_stream_')
+  assert names['mfma_fp32'].startswith(base + '_mfma_fp32_')
+  assert names['mfma_bf16x3'].startswith(base + '_mfma_bf16x3_')
+  assert len(set(names.values())) == 4
