@@ -154,3 +154,40 @@ def call_site_maps(descrs):
   """Name maps for ``Generator.generate_call_site``: ``<alias>_ptr`` / ``<alias>_off``."""
   names = aliases(descrs)
   return ({a: f'{a}_ptr' for a in names}, {a: f'{a}_off' for a in names})
+
+
+def extra_cases():
+  """Cases beyond the name set whose numbers are pinned by the OpenMP fixtures
+  (``make_omp_fixtures.py``): the ADER volume chain in FP32 and at order 7 (84 basis functions,
+  dense K, the precision study's shape)."""
+  from tensorforge.common.aux import generate_tmp_tensor
+  from tensorforge.common.basic_types import Addressing
+  from tensorforge.common.matrix.spp import ListSPP
+  from tensorforge.common.matrix.tensor import SubTensor, Tensor
+  from tensorforge.generators.descriptions import MultilinearDescr
+
+  pair = [[0, -1], [-1, 1]]
+  ident = [[0, 1], [0, 1]]
+
+  def ader(sparse_k, b, q=9):
+    rng = np.random.default_rng(1)
+    masks = rng.random((3, b, b)) < 0.4
+    Q = Tensor([b, q], Addressing.STRIDED, alias='Q')
+    Qp = Tensor([b, q], Addressing.STRIDED, alias='Qp')
+    descrs = []
+    for d in range(3):
+      S = Tensor([q, q], Addressing.STRIDED, spp=ListSPP(STAR, (q, q)), alias=f'S{d + 1}')
+      if sparse_k:
+        K = Tensor([b, b], Addressing.NONE, spp=ListSPP(_csr(masks[d]), (b, b)),
+                   alias=f'K{d + 1}')
+      else:
+        K = Tensor([b, b], Addressing.NONE, alias=f'K{d + 1}')
+      qs, ss, ks = SubTensor(Q), SubTensor(S), SubTensor(K)
+      T = generate_tmp_tensor([qs, ss], pair)
+      descrs.append(MultilinearDescr(SubTensor(T), [qs, ss], pair, ident))
+      descrs.append(MultilinearDescr(SubTensor(Qp), [ks, SubTensor(T)], pair, ident,
+                                     add=d > 0))
+    return descrs
+
+  return [('ader_volume_o6_sparse_float', 'float', ader(True, 56)),
+          ('ader_volume_o7_dense', 'double', ader(False, 84))]
