@@ -35,6 +35,9 @@ def parse():
   ap.add_argument('--elements', type=int, default=262144)
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--cpu-seconds', type=float, default=25.0)
+  ap.add_argument('--cpu-full', action='store_true',
+                  help='time the CPU baseline over the whole batch (minutes) instead of a '
+                       'bounded sample')
   ap.add_argument('--only', default='',
                   help='comma list of legs (tuned,plain,f32,copy,mixed,precision,flux,deriv,'
                        'tuner,pointwise,gemm)')
@@ -63,27 +66,45 @@ def make_inputs(entry, n, device, dtype, seed=2):
   return bufs
 
 
-def time_kernel(kernel, bufs, n, steps, warmup):
-  """Average launch duration (s) over ``steps`` launches, HIP events on the launch stream."""
+def step_times(fn, steps, warmup, stream):
+  """Durations (s) of ``steps`` calls of ``fn`` after ``warmup`` untimed ones: one HIP event on
+  ``stream`` before the first timed call and one after each, so consecutive events bracket one
+  step each."""
+  import torch
+  for _ in range(warmup):
+    fn()
+  torch.cuda.synchronize()
+  events = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+  events[0].record(stream)
+  for i in range(steps):
+    fn()
+    events[i + 1].record(stream)
+  torch.cuda.synchronize()
+  return [events[i].elapsed_time(events[i + 1]) / 1e3 for i in range(steps)]
+
+
+def median(xs):
+  ys = sorted(xs)
+  m = len(ys) // 2
+  return ys[m] if len(ys) % 2 else 0.5 * (ys[m - 1] + ys[m])
+
+
+def launch_times(kernel, bufs, n, steps, warmup):
+  """Per-launch durations (s) of the launcher on the current stream."""
   import torch
   from tensorforge_amd.device import launch_args
   stream = torch.cuda.current_stream()
   args = launch_args(kernel.generator, bufs, n, stream=stream)
-  for _ in range(warmup):
-    kernel(*args)
-  torch.cuda.synchronize()
-  start = torch.cuda.Event(enable_timing=True)
-  stop = torch.cuda.Event(enable_timing=True)
-  start.record(stream)
-  for _ in range(steps):
-    kernel(*args)
-  stop.record(stream)
-  torch.cuda.synchronize()
-  return start.elapsed_time(stop) / 1e3 / steps
+  return step_times(lambda: kernel(*args), steps, warmup, stream)
 
 
-def copy_bandwidth(device, steps=20):
-  """Measured HBM roof: 2 * bytes / time of a 16-byte-vector copy of 2 GiB."""
+def time_kernel(kernel, bufs, n, steps, warmup):
+  """Median per-launch duration (s) over ``steps`` launches after ``warmup``."""
+  return median(launch_times(kernel, bufs, n, steps, warmup))
+
+
+def copy_bandwidth(device, steps, warmup):
+  """Measured HBM roof: 2 * bytes / median launch time of a 16-byte-vector copy of 2 GiB."""
   import torch
   from tensorforge_amd.runtime import runtime
   rt = runtime()
@@ -91,17 +112,9 @@ def copy_bandwidth(device, steps=20):
   src = torch.empty(n, dtype=torch.float64, device=device).normal_()
   dst = torch.empty_like(src)
   stream = torch.cuda.current_stream()
-  for _ in range(3):
-    rt.tensorforge_copy(dst.data_ptr(), src.data_ptr(), n, stream.cuda_stream)
-  torch.cuda.synchronize()
-  start = torch.cuda.Event(enable_timing=True)
-  stop = torch.cuda.Event(enable_timing=True)
-  start.record(stream)
-  for _ in range(steps):
-    rt.tensorforge_copy(dst.data_ptr(), src.data_ptr(), n, stream.cuda_stream)
-  stop.record(stream)
-  torch.cuda.synchronize()
-  t = start.elapsed_time(stop) / 1e3 / steps
+  t = median(step_times(
+    lambda: rt.tensorforge_copy(dst.data_ptr(), src.data_ptr(), n, stream.cuda_stream),
+    steps, warmup, stream))
   del src, dst
   return 2 * n * 8 / t / 1e9
 
@@ -158,20 +171,10 @@ def mixed_leg(device, steps, warmup, graph_steps=10):
 
   def timed(fn, n_steps, n_warm):
     with torch.cuda.stream(stream):
-      for _ in range(n_warm):
-        fn()
-      torch.cuda.synchronize()
-      t0 = torch.cuda.Event(enable_timing=True)
-      t1 = torch.cuda.Event(enable_timing=True)
-      t0.record(stream)
-      for _ in range(n_steps):
-        fn()
-      t1.record(stream)
-      torch.cuda.synchronize()
-    return t0.elapsed_time(t1) / 1e3 / n_steps
+      return median(step_times(fn, n_steps, n_warm, stream))
 
   res = {'elements': total, 'counts': dict(zip((str(b) for b in W.MIXED_BASES), counts))}
-  t_plain = timed(lambda: [k(*a) for k, a in zip(plain, args[0])], max(2, steps // 10), 1)
+  t_plain = timed(lambda: [k(*a) for k, a in zip(plain, args[0])], steps, warmup)
   t_tuned = timed(lambda: [k(*a) for k, a in zip(tuned, args[0])], steps, warmup)
   t_group = timed(lambda: gk(*gargs[0]), steps, warmup)
 
@@ -188,7 +191,7 @@ def mixed_leg(device, steps, warmup, graph_steps=10):
   for name, fn in (('grouped', seq_grouped), ('per_order', seq_members)):
     gr = Graph(stream)
     gr.capture(fn)
-    t = timed(gr.launch, max(2, steps // graph_steps), 1) / graph_steps
+    t = timed(gr.launch, max(2, steps // graph_steps), max(1, warmup // graph_steps)) / graph_steps
     graphs[name] = t
     gr.close()
   res.update({
@@ -264,7 +267,7 @@ def flux_leg(device, steps, warmup, n=262144):
     bufs.pop('__keep__')
     bpe = tuned.generator.get_emitted().bytes_per_element
     t_t = time_kernel(tuned, bufs, n, steps, warmup)
-    t_p = time_kernel(plain, bufs, n, max(3, steps // 5), 2)
+    t_p = time_kernel(plain, bufs, n, steps, warmup)
     res[kind] = {'tuned': {'value': n / t_t, 'ms_per_step': t_t * 1e3, 'hbm_GBps': bpe * n / t_t / 1e9},
                  'plain': {'value': n / t_p, 'ms_per_step': t_p * 1e3},
                  'tuned_over_plain': t_p / t_t, 'bytes_per_element': bpe}
@@ -296,8 +299,10 @@ def deriv_leg(device, steps, warmup, n=262144):
     bufs.pop('__keep__')
     em = tuned.generator.get_emitted()
     bpe = em.bytes_per_element
-    t_t = time_kernel(tuned, bufs, n, steps if em.schedule.get('kind') == 'stream' else 3, 2)
-    t_p = time_kernel(plain, bufs, n, max(3, steps // 5), 2)
+    # the column-per-lane form of the not-aligned chain takes ~30 ms per launch: fewer launches
+    stream_form = em.schedule.get('kind') == 'stream'
+    t_t = time_kernel(tuned, bufs, n, steps if stream_form else 3, warmup if stream_form else 2)
+    t_p = time_kernel(plain, bufs, n, steps, warmup)
     t_d = t_t if choice == 'tuned' else t_p
     form = 'streaming' if em.schedule.get('kind') == 'stream' else 'column-per-lane'
     res[label] = {'default': {'emitter': choice if choice == 'plain' else form,
@@ -341,7 +346,7 @@ def gemm_leg(device, steps, warmup, cpu_seconds, with_cpu, n=65536):
     got = bufs['C'].cpu().numpy().reshape(n, q, b)
     err = float((np.abs(got - want).reshape(n, -1).max(axis=1) /
                  np.abs(want).reshape(n, -1).max(axis=1)).max())
-    t = time_kernel(k, bufs, n, steps if label == 'tuned' else max(3, steps // 5), warmup)
+    t = time_kernel(k, bufs, n, steps, warmup)
     res[label] = {'value': n / t, 'ms_per_step': t * 1e3, 'hbm_GBps': bpl / t / 1e9,
                   'frac_of_peak': bpl / t / 1e9 / HBM_PEAK_GBPS,
                   'max_normwise_rel_err_vs_closed_form': err}
@@ -351,7 +356,8 @@ def gemm_leg(device, steps, warmup, cpu_seconds, with_cpu, n=65536):
   if with_cpu:
     lib = ctypes.CDLL(os.path.join(ROOT, 'oracle', 'lib', 'libomp_chain.so'))
     DP = ctypes.POINTER(ctypes.c_double)
-    cpu = {'cores': lib.omp_max_threads(), 'kind': 'port'}
+    cpu = {'cores': lib.omp_max_threads(), 'kind': 'port',
+           'host': host_info(lib.omp_max_threads())}
     for form in ('team', 'flat'):
       m = 256
       while True:
@@ -450,7 +456,24 @@ def profiled_traffic(kernel_name, schedule):
   return int(fields['traffic_bytes_per_launch'])
 
 
-def cpu_baseline(entry, bufs, gpu_out, budget_s):
+def host_info(threads):
+  """The host the CPU baseline ran on: CPU model, logical CPUs, CPUs this process may use, the
+  OpenMP thread count and every OMP_* setting of the environment."""
+  model = None
+  try:
+    with open('/proc/cpuinfo', encoding='utf-8') as f:
+      for line in f:
+        if line.startswith('model name'):
+          model = line.split(':', 1)[1].strip()
+          break
+  except OSError:
+    pass
+  return {'cpu_model': model, 'logical_cpus': os.cpu_count(),
+          'cpus_usable': len(os.sched_getaffinity(0)), 'omp_threads': threads,
+          'omp_env': {k: v for k, v in sorted(os.environ.items()) if k.startswith('OMP_')}}
+
+
+def cpu_baseline(entry, bufs, gpu_out, budget_s, full=False):
   """The OpenMP restatement of the reference's generated kernel (oracle/omp_chain.cpp, team
   form and flat form) on the first elements of the same inputs, sized to ~budget_s of CPU work;
   also the GPU result's max normwise error against it on that sample."""
@@ -495,14 +518,18 @@ def cpu_baseline(entry, bufs, gpu_out, budget_s):
                         ctypes.c_long(n))
     return time.perf_counter() - t0, out
 
-  result = {'cores': threads, 'kind': 'port'}
+  total = len(Q) // (b * q)
+  result = {'cores': threads, 'kind': 'port', 'host': host_info(threads)}
   for form in ('team', 'flat'):
-    n = 256
-    t, _ = run(form, n)
-    while t < budget_s / 8 and n < 262144:
-      n = min(262144, n * 4)
+    if full:
+      n = total
+    else:
+      n = 256
       t, _ = run(form, n)
-    n = int(min(262144, max(256, n * budget_s / 2 / max(t, 1e-6))) // 8 * 8)
+      while t < budget_s / 8 and n < total:
+        n = min(total, n * 4)
+        t, _ = run(form, n)
+      n = int(min(total, max(256, n * budget_s / 2 / max(t, 1e-6))) // 8 * 8)
     t, out = run(form, n)
     result[form] = {'value': n / t, 'elements': n, 'seconds': t}
     if form == 'flat':
@@ -535,23 +562,18 @@ def main():
   emitted = tuned.generator.get_emitted()
   bytes_per_launch = emitted.bytes_per_element * n + emitted.shared_bytes
   flops = tuned.generator.get_program().flops_per_element() * n
-  t_tuned = time_kernel(tuned, bufs, n, args.steps, args.warmup)
-  gpu_out = bufs['Qp'].cpu().numpy()
-  value = n / t_tuned
-  achieved = bytes_per_launch / t_tuned / 1e9
+
+  # the copy probe and the plain kernels run before the headline kernel is timed: the GPU is
+  # then at its loaded clocks, as in a time-stepping run (a GPU idle before the first launch
+  # runs its first few dozen launches slower)
+  measured_bw = copy_bandwidth(device, args.steps, args.warmup) if 'copy' in legs else None
 
   if 'plain' in legs:
     _, plain = catalog.load('ader_f64_plain')
     bufs_plain = dict(bufs)
     bufs_plain['Qp'] = torch.zeros_like(bufs['Qp'])
-    t_plain = time_kernel(plain, bufs_plain, n, max(3, args.steps // 5), 2)
+    t_plain = time_kernel(plain, bufs_plain, n, args.steps, args.warmup)
     plain_out = bufs_plain['Qp'].cpu().numpy().reshape(n, -1)
-    got = gpu_out.reshape(n, -1)
-    agree = float((np.abs(got - plain_out).max(axis=1) /
-                   np.maximum(np.abs(plain_out).max(axis=1), 1e-300)).max())
-    out['plain'] = {'value': n / t_plain, 'ms_per_step': t_plain * 1e3,
-                    'tuned_over_plain': t_plain / t_tuned,
-                    'max_normwise_rel_diff_tuned_vs_plain': agree}
     del bufs_plain
     _, plain_dense = catalog.load('ader_dense_f64_plain')
     bufs_pd = dict(bufs)
@@ -564,10 +586,27 @@ def main():
       dense[idx] = bufs[f'K{d + 1}']
       bufs_pd[f'K{d + 1}'] = dense
     bufs_pd['Qp'] = torch.zeros_like(bufs['Qp'])
-    t_pd = time_kernel(plain_dense, bufs_pd, n, max(3, args.steps // 5), 2)
+    t_pd = time_kernel(plain_dense, bufs_pd, n, args.steps, args.warmup)
+    del bufs_pd
+
+  times = launch_times(tuned, bufs, n, args.steps, args.warmup)
+  t_tuned = median(times)
+  gpu_out = bufs['Qp'].cpu().numpy()
+  value = n / t_tuned
+  achieved = bytes_per_launch / t_tuned / 1e9
+  timing = {'statistic': 'median of per-launch HIP-event durations on the launch stream',
+            'median_ms': t_tuned * 1e3, 'mean_ms': sum(times) / len(times) * 1e3,
+            'min_ms': min(times) * 1e3, 'max_ms': max(times) * 1e3}
+
+  if 'plain' in legs:
+    got = gpu_out.reshape(n, -1)
+    agree = float((np.abs(got - plain_out).max(axis=1) /
+                   np.maximum(np.abs(plain_out).max(axis=1), 1e-300)).max())
+    out['plain'] = {'value': n / t_plain, 'ms_per_step': t_plain * 1e3,
+                    'tuned_over_plain': t_plain / t_tuned,
+                    'max_normwise_rel_diff_tuned_vs_plain': agree}
     out['plain_dense_k'] = {'value': n / t_pd, 'ms_per_step': t_pd * 1e3,
                             'tuned_over_plain_dense_k': t_pd / t_tuned}
-    del bufs_pd
 
   if 'f32' in legs:
     e32, k32 = catalog.load('ader_f32_tuned')
@@ -586,7 +625,7 @@ def main():
     out['mixed_orders'] = mixed_leg(device, args.steps, args.warmup)
 
   if 'precision' in legs:
-    out['precision_order7'] = precision_leg(device, max(3, args.steps // 5), 2)
+    out['precision_order7'] = precision_leg(device, args.steps, args.warmup)
 
   if 'flux' in legs:
     out['flux'] = flux_leg(device, args.steps, args.warmup)
@@ -603,11 +642,9 @@ def main():
   if 'tuner' in legs:
     out['cost_model'] = tuner_leg()
 
-  measured_bw = copy_bandwidth(device) if 'copy' in legs else None
-
   cpu = None
   if not args.no_cpu:
-    cpu = cpu_baseline(entry, bufs, gpu_out, args.cpu_seconds)
+    cpu = cpu_baseline(entry, bufs, gpu_out, args.cpu_seconds, full=args.cpu_full)
 
   line = {
     'metric': 'elements/s + % HBM roofline, fused ADER chain, tuned vs plain at stated rel err',
@@ -617,6 +654,7 @@ def main():
     'steps': args.steps,
     'warmup': args.warmup,
     'ms_per_step': t_tuned * 1e3,
+    'timing': timing,
     'higher_is_better': True,
     'scaling': 'weak',
     'vs_baseline': None,
@@ -640,8 +678,10 @@ def main():
   if cpu is not None:
     line['cpu_baseline'] = {
       'value': cpu['team']['value'], 'unit': 'elements/s', 'cores': cpu['cores'], 'kind': 'port',
-      'sample': (f"first {cpu['team']['elements']} elements of the same inputs, OpenMP team form "
-                 f"(reference omptarget structure: 32 lanes x 8 elements per team)"),
+      'sample': (f"first {cpu['team']['elements']} of {n} elements of the same inputs, OpenMP "
+                 f"team form (reference omptarget structure: 32 lanes x 8 elements per team), "
+                 f"{cpu['team']['seconds']:.1f} s"),
+      'host': cpu['host'],
       'flat_form': cpu['flat'],
       'gpu_max_normwise_rel_err_vs_cpu': cpu.get('gpu_max_normwise_rel_err')}
   print(json.dumps(line))
