@@ -17,11 +17,15 @@ contraction rows and per term, each lane forms its column of the temporary (FP64
 columns through ``ds_bpermute``), rounds or splits it into a per-wavefront LDS tile in B-operand
 order, and the wavefront issues the MFMAs of that block.
 
-Split BF16x3 (the three-product scheme): ``x = hi + lo`` with ``hi = bf16(x)``,
-``lo = bf16(x - hi)``; ``a·b ~ hi_a·hi_b + (hi_a·lo_b + lo_a·hi_b)``; the first product
-accumulates in one FP32 accumulator, the two correction products in a second, and the two are
-added in FP64 at the end, so the small corrections are not absorbed by the rounding of the large
-sum.
+Split BF16x3 (three-term split, six products; the scheme of the reference's unused design input
+``backend/instructions/compute/primitives/nvidia.py:126-170``): both operands are kept in FP32 in
+LDS and split into three BF16 terms when a fragment is read, ``x = hi + mid + lo`` with
+``hi = rne_bf16(x)``, ``mid = rne_bf16(x - hi)``, ``lo = x - hi - mid``. Both differences are exact
+in FP32, and ``lo`` has at most 8 significant bits, so the three terms hold the FP32 value exactly.
+``a·b ~ hi_a·hi_b + (hi_a·mid_b + mid_a·hi_b + hi_a·lo_b + lo_a·hi_b + mid_a·mid_b)``: the dropped
+``mid·lo``, ``lo·mid`` and ``lo·lo`` products are below 2^-24 of the leading one. The first product
+accumulates in one FP32 accumulator and the five corrections in a second. The two are added in FP64
+at the end, so the small corrections are not absorbed by the rounding of the large sum.
 """
 from ..common.basic_types import Addressing, FloatingPointType
 from . import ir
@@ -81,7 +85,7 @@ class _Mfma:
     self.RT = (self.rows + 15) // 16
     self.K = pl.khi - pl.klo
     self.NKB = (self.K + KB - 1) // KB
-    self.kstep = 4 if mode == 'fp32' else 16
+    self.kstep = 4 if mode == 'fp32' else 16      # k per MFMA: 16x16x4 f32, 16x16x16 bf16
     self.KS = self.NKB * (KB // self.kstep)
     self.T = len(pl.terms)
     self.prefetch_rows = mode != 'bf16x3'
@@ -129,25 +133,44 @@ class _Mfma:
     threads = WAVE * WAVES
     frag = T * KS * RT * WAVE
     if bf16:
-      kbytes = frag * 8 * 2
-      pitch = 20                          # bf16 per column of the B tile (conflict-free reads)
-      tbytes = 64 * pitch * 2 * 2
+      # FP32 in LDS, split into BF16 terms when read: per lane and fragment the 4 consecutive
+      # contraction entries of its row (A) or column (B), one 16-byte read
+      kbytes = frag * 16
+      pitch = 20                          # floats per column of the B tile (16-byte rows)
     else:
       kbytes = frag * 4
       pitch = 17
-      tbytes = 64 * pitch * 4
+    tbytes = 64 * pitch * 4
     lds_bytes = kbytes + WAVES * 2 * tbytes
     self.lds_bytes = lds_bytes
     w('typedef float tf_f4 __attribute__((ext_vector_type(4)));')
     w('typedef short tf_s4 __attribute__((ext_vector_type(4)));')
-    w('#ifndef TF_BF16_HELPERS')
-    w('#define TF_BF16_HELPERS')
-    w('__device__ __forceinline__ unsigned tf_bf16(float x) {')
-    w('  unsigned u = __float_as_uint(x);')
-    w('  u += 0x7fffu + ((u >> 16) & 1u);')
-    w('  return u >> 16;')
+    w('#ifndef TF_BF16_SPLIT3')
+    w('#define TF_BF16_SPLIT3')
+    # tf_rne_top: an FP32 value rounded to BF16 (nearest, ties to even), kept in the top 16 bits
+    # (finite inputs). tf_split3: 4 FP32 values split into BF16 hi / mid / lo fragments (entry j
+    # in halfword j)
+    w('__device__ __forceinline__ unsigned tf_rne_top(float x) {')
+    w('  const unsigned u = __float_as_uint(x);')
+    w('  return (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;')
     w('}')
-    w('__device__ __forceinline__ float tf_bf16_f(unsigned h) { return __uint_as_float(h << 16); }')
+    w('__device__ __forceinline__ void tf_split3(const float4 x, tf_s4& h, tf_s4& m, tf_s4& l) {')
+    w('  const float v[4] = {x.x, x.y, x.z, x.w};')
+    w('  unsigned a[4], b[4], c[4];')
+    w('#pragma unroll')
+    w('  for (int j = 0; j < 4; ++j) {')
+    w('    a[j] = tf_rne_top(v[j]);')
+    w('    const float r = v[j] - __uint_as_float(a[j]);')
+    w('    b[j] = tf_rne_top(r);')
+    w('    c[j] = __float_as_uint(r - __uint_as_float(b[j]));')
+    w('  }')
+    w('  h = __builtin_bit_cast(tf_s4, make_uint2((a[0] >> 16) | (a[1] & 0xffff0000u), '
+      '(a[2] >> 16) | (a[3] & 0xffff0000u)));')
+    w('  m = __builtin_bit_cast(tf_s4, make_uint2((b[0] >> 16) | (b[1] & 0xffff0000u), '
+      '(b[2] >> 16) | (b[3] & 0xffff0000u)));')
+    w('  l = __builtin_bit_cast(tf_s4, make_uint2((c[0] >> 16) | (c[1] & 0xffff0000u), '
+      '(c[2] >> 16) | (c[3] & 0xffff0000u)));')
+    w('}')
     w('#endif')
     params = ', '.join(kernel_params(self.prog, self.ctx))
     w(f'extern "C" __global__ void __launch_bounds__({threads}, 1) {self.kname}({params})')
@@ -157,19 +180,14 @@ class _Mfma:
       f'columns each, {RT}x{CT} tiles of 16x16, contraction blocks of {KB}')
     w('extern __shared__ __align__(16) unsigned char lds_raw[];')
     if bf16:
-      w('uint2* __restrict__ kh = reinterpret_cast<uint2*>(lds_raw);')
-      w(f'uint2* __restrict__ kl = kh + {frag};')
+      w('float4* __restrict__ kf4 = reinterpret_cast<float4*>(lds_raw);')
     else:
       w('float* __restrict__ kf = reinterpret_cast<float*>(lds_raw);')
     w('const int wave = threadIdx.x / 64;')
     w('const int lane = threadIdx.x % 64;')
     for b in range(2):
       base = f'lds_raw + {kbytes} + (wave * 2 + {b}) * {tbytes}'
-      if bf16:
-        w(f'unsigned short* __restrict__ th{b} = reinterpret_cast<unsigned short*>({base});')
-        w(f'unsigned short* __restrict__ tl{b} = th{b} + {64 * pitch};')
-      else:
-        w(f'float* __restrict__ tf{b} = reinterpret_cast<float*>({base});')
+      w(f'float* __restrict__ tf{b} = reinterpret_cast<float*>({base});')
     self.stage_shared(frag, threads)
     w('__syncthreads();')
     q, G = self.q, self.G
@@ -232,13 +250,11 @@ class _Mfma:
       if self.mode == 'fp32':
         w(f'kf[e] = (float){self.shared_value("t", "row", "ks * 4 + l / 16")};')
       else:
-        w('unsigned hi[4], lo[4];')
+        w('float x[4];')
+        w('#pragma unroll')
         with w.block('for (int j = 0; j < 4; ++j)'):
-          w(f'const float x = (float){self.shared_value("t", "row", "ks * 16 + 4 * (l / 16) + j")};')
-          w('hi[j] = tf_bf16(x);')
-          w('lo[j] = tf_bf16(x - tf_bf16_f(hi[j]));')
-        w('kh[e] = make_uint2(hi[0] | (hi[1] << 16), hi[2] | (hi[3] << 16));')
-        w('kl[e] = make_uint2(lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16));')
+          w(f'x[j] = (float){self.shared_value("t", "row", "ks * 16 + 4 * (l / 16) + j")};')
+        w('kf4[e] = make_float4(x[0], x[1], x[2], x[3]);')
 
   def group(self, pitch):
     w = self.w
@@ -332,12 +348,7 @@ class _Mfma:
       for si, (src, pv) in enumerate(tabs):
         w(f'  {v} = __builtin_fma(__hiloint2double(__builtin_amdgcn_ds_bpermute({src}, qhi), '
           f'__builtin_amdgcn_ds_bpermute({src}, qlo)), w{ti}_{si}, {v});')
-      if self.mode == 'bf16x3':
-        w(f'  const float tx = (float){v}; const unsigned h = tf_bf16(tx);')
-        w(f'  th{tile}[gcol * {pitch} + {j}] = (unsigned short)h;')
-        w(f'  tl{tile}[gcol * {pitch} + {j}] = (unsigned short)tf_bf16(tx - tf_bf16_f(h)); }}')
-      else:
-        w(f'  tf{tile}[gcol * {pitch} + {j}] = (float){v}; }}')
+      w(f'  tf{tile}[gcol * {pitch} + {j}] = (float){v}; }}')
 
   def mfma_list(self, kb, ti, tile, pitch):
     """Source lines of step (kb, ti)'s MFMAs from B tile ``tile``: fragment loads, then the
@@ -350,10 +361,10 @@ class _Mfma:
       for ct in range(CT):
         if self.mode == 'bf16x3':
           o = f'(({ct * 16} + lane % 16) * {pitch} + 4 * (lane / 16))'
-          out.append(f'const tf_s4 bh{ks}_{ct}{sfx} = __builtin_bit_cast(tf_s4, '
-                     f'*reinterpret_cast<const uint2*>(th{tile} + {o}));')
-          out.append(f'const tf_s4 bl{ks}_{ct}{sfx} = __builtin_bit_cast(tf_s4, '
-                     f'*reinterpret_cast<const uint2*>(tl{tile} + {o}));')
+          b = f'{ks}_{ct}{sfx}'
+          out.append(f'tf_s4 bh{b}, bm{b}, bl{b};')
+          out.append(f'tf_split3(*reinterpret_cast<const float4*>(tf{tile} + {o}), bh{b}, bm{b}, '
+                     f'bl{b});')
         else:
           out.append(f'const float bv{ks}_{ct}{sfx} = tf{tile}[({ct * 16} + lane % 16) * {pitch} + '
                      f'{ks * 4} + lane / 16];')
@@ -361,20 +372,21 @@ class _Mfma:
       for rt in range(RT):
         frag = f'{((ti * self.KS + kb * nks + ks) * RT + rt) * 64} + lane'
         if self.mode == 'bf16x3':
-          out.append(f'const tf_s4 ah{ks}_{rt}{sfx} = __builtin_bit_cast(tf_s4, kh[{frag}]);')
-          out.append(f'const tf_s4 al{ks}_{rt}{sfx} = __builtin_bit_cast(tf_s4, kl[{frag}]);')
+          a = f'{ks}_{rt}{sfx}'
+          out.append(f'tf_s4 ah{a}, am{a}, al{a};')
+          out.append(f'tf_split3(kf4[{frag}], ah{a}, am{a}, al{a});')
         else:
           out.append(f'const float av{ks}_{rt}{sfx} = kf[{frag}];')
         for ct in range(CT):
           a = f'acc[{rt * CT + ct}]'
           if self.mode == 'bf16x3':
             r = f'acr[{rt * CT + ct}]'
-            out.append(f'{a} = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah{ks}_{rt}{sfx}, '
-                       f'bh{ks}_{ct}{sfx}, {a}, 0, 0, 0);')
-            out.append(f'{r} = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah{ks}_{rt}{sfx}, '
-                       f'bl{ks}_{ct}{sfx}, {r}, 0, 0, 0);')
-            out.append(f'{r} = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al{ks}_{rt}{sfx}, '
-                       f'bh{ks}_{ct}{sfx}, {r}, 0, 0, 0);')
+            ai, bi = f'{ks}_{rt}{sfx}', f'{ks}_{ct}{sfx}'
+            mf = '__builtin_amdgcn_mfma_f32_16x16x16bf16_1k'
+            out.append(f'{a} = {mf}(ah{ai}, bh{bi}, {a}, 0, 0, 0);')
+            # corrections, smallest first: mid·mid, lo·hi, hi·lo, mid·hi, hi·mid
+            for x, y in (('am', 'bm'), ('al', 'bh'), ('ah', 'bl'), ('am', 'bh'), ('ah', 'bm')):
+              out.append(f'{r} = {mf}({x}{ai}, {y}{bi}, {r}, 0, 0, 0);')
           else:
             out.append(f'{a} = __builtin_amdgcn_mfma_f32_16x16x4f32(av{ks}_{rt}{sfx}, '
                        f'bv{ks}_{ct}{sfx}, {a}, 0, 0, 0);')

@@ -5,7 +5,7 @@ workgroups and loop over element groups, carrying prefetched rows from one group
 The parity tests' batches fit in one round; here every tuned launcher runs 60,001 elements
 (at least two rounds on MI355X, a partial last group) and is compared on every element with the
 plain launcher of the same chain, which has no cross-group state (itself checked against the
-oracle). Normwise per element: 1e-12 FP64, 1e-5 FP32 and FP32 MFMA, 1e-4 split BF16x3.
+oracle). Normwise per element: 1e-12 FP64, 1e-5 FP32, FP32 MFMA and split BF16x3.
 """
 import pytest
 
@@ -20,7 +20,7 @@ N = 60001
 PAIRS = [(t, t.rsplit('_', 1)[0] + '_plain', 1e-5 if '_f32_' in t else 1e-12)
          for t in catalog.names() if t.endswith('_tuned') and not t.startswith('t_')]
 PAIRS += [('ader7_f64_mfma_fp32', 'ader7_f64_plain', 1e-5),
-          ('ader7_f64_mfma_bf16x3', 'ader7_f64_plain', 1e-4)]
+          ('ader7_f64_mfma_bf16x3', 'ader7_f64_plain', 1e-5)]
 PAIRS += [(t, t[:-len('_auto')] + '_plain',
            1e-5 if catalog.get(t).fp_type == FloatingPointType.FLOAT else 1e-12)
           for t in catalog.names() if t.startswith('t_') and t.endswith('_auto')]
