@@ -352,6 +352,70 @@ def _five_entry():
                                  'D': md.tensor})
 
 
+@register('t_three_matrices_plain')
+def _three_matrices_entry():
+  """D += A x (B x C), all strided (reference example/three_matrices.py:9-27)."""
+  md = _sub(_t([56, 9], Addressing.STRIDED, ([0, 0], [56, 9]), alias='D'), ([0, 0], [56, 9]))
+  ma = _sub(_t([56, 56], Addressing.STRIDED, ([0, 0], [56, 56]), alias='A'), ([0, 0], [56, 56]))
+  mb = _sub(_t([56, 9], Addressing.STRIDED, ([0, 0], [56, 9]), alias='B'), ([0, 0], [56, 9]))
+  mc = _sub(_t([9, 9], Addressing.STRIDED, ([0, 0], [9, 9]), alias='C'), ([0, 0], [9, 9]))
+  tmp1 = generate_tmp_matrix(mb, mc)
+  descrs = [GemmDescr(False, False, mb, mc, _sub(tmp1)),
+            GemmDescr(False, False, ma, _sub(tmp1), md, alpha=1.0, beta=1.0)]
+  return Entry('', descrs, F32, {'D': md.tensor, 'A': ma.tensor, 'B': mb.tensor,
+                                 'C': mc.tensor})
+
+
+def _a_atrans_bc(addr_a, bq, bc, addr_c):
+  """Q += A x ((A^T x B) x C) over 20-row windows of 56x56 tensors."""
+  box = ([0, 0], bq)
+  mq = _sub(_t([56, 56], Addressing.STRIDED, box, alias='Q'), box)
+  ma = _sub(_t([56, 56], addr_a, box, alias='A'), box)
+  mb = _sub(_t([56, 56], Addressing.STRIDED, box, alias='B'), box)
+  mc = _sub(_t([56, 9], addr_c, ([0, 0], bc), alias='C'), ([0, 0], bc))
+  tmp1 = generate_tmp_matrix(ma, mb, True, False)
+  tmp2 = generate_tmp_matrix(tmp1, mc)
+  descrs = [GemmDescr(True, False, ma, mb, _sub(tmp1)),
+            GemmDescr(False, False, _sub(tmp1), mc, _sub(tmp2)),
+            GemmDescr(False, False, ma, _sub(tmp2), mq, alpha=1.0, beta=1.0)]
+  return Entry('', descrs, F32, {'Q': mq.tensor, 'A': ma.tensor, 'B': mb.tensor,
+                                 'C': mc.tensor})
+
+
+# reference example/a_atrans_bc.py:27-42 (variant v0: A strided)
+register('t_a_atrans_bc_plain')(
+  lambda: _a_atrans_bc(Addressing.STRIDED, [20, 9], [9, 9], Addressing.STRIDED))
+# reference example/redundant_load_opt.py:23-40 (A and C shared by all elements)
+register('t_redundant_load_opt_plain')(
+  lambda: _a_atrans_bc(Addressing.NONE, [20, 8], [8, 8], Addressing.NONE))
+
+
+@register('t_ab_transb_plain')
+def _ab_transb_entry():
+  """Q += (A x B^T) x B, Q pointer-based, A shared (reference example/ab_transb.py:19-32)."""
+  box = ([0, 0], [20, 9])
+  mq = _sub(_t([56, 56], Addressing.PTR_BASED, box, alias='Q'), box)
+  ma = _sub(_t([56, 56], Addressing.NONE, box, alias='A'), box)
+  mb = _sub(_t([56, 56], Addressing.STRIDED, box, alias='B'), box)
+  tmp1 = generate_tmp_matrix(ma, mb, trans_a=False, trans_b=True)
+  descrs = [GemmDescr(False, True, ma, mb, _sub(tmp1)),
+            GemmDescr(False, False, _sub(tmp1), mb, mq, alpha=1.0, beta=1.0)]
+  return Entry('', descrs, F32, {'Q': mq.tensor, 'A': ma.tensor, 'B': mb.tensor})
+
+
+@register('t_abb_trans_plain')
+def _abb_trans_entry():
+  """Q += (A x B) x B^T, Q pointer-based, A shared (reference example/abb_trans.py:21-34)."""
+  box = ([0, 0], [20, 9])
+  mq = _sub(_t([56, 56], Addressing.PTR_BASED, box, alias='Q'), box)
+  ma = _sub(_t([56, 56], Addressing.NONE, box, alias='A'), box)
+  mb = _sub(_t([56, 56], Addressing.STRIDED, ([0, 0], [9, 20]), alias='B'), ([0, 0], [9, 20]))
+  tmp1 = generate_tmp_matrix(ma, mb)
+  descrs = [GemmDescr(False, False, ma, mb, _sub(tmp1)),
+            GemmDescr(False, True, _sub(tmp1), mb, mq, alpha=1.0, beta=1.0)]
+  return Entry('', descrs, F32, {'Q': mq.tensor, 'A': ma.tensor, 'B': mb.tensor})
+
+
 @register('t_mat_vec_plain')
 def _mat_vec_entry():
   """C += A x b (reference example/mat_vec.py:9-20)."""
