@@ -480,6 +480,10 @@ class _Stream:
              'dpp': self.sched.dpp}
     if self.loop:
       sched['loop'] = True
+    if self.pair:
+      sched['pair'] = True
+    if pl.L is not None and self.row_loads():
+      sched['row_loads'] = True
     if self.prog.epilogue is not None:
       sched['epilogue'] = 'pointwise'
     if self.dma:

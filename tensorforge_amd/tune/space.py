@@ -1,10 +1,19 @@
 """Candidate schedules of a chain and the random contraction chains the cost model learns from.
 
-Candidates: the plain emitter (the reference's thread mapping), the column-per-lane tuned
-emitter with a few workgroup sizes and contraction blocks, and the streaming schedule with a few
-workgroup sizes, contraction blocks and LDS read-ahead depths. A candidate that the chain's
-shape does not admit (the emitter falls back to another form) is dropped, so every candidate
-label names the kernel that is actually built.
+Candidates (tile shapes, LDS layout and the threads-per-element schedule):
+
+* the plain emitter (the reference's thread mapping: a row of lanes per element);
+* the column-per-lane tuned emitter (one lane per output column) with 1 or 4 wavefronts per
+  workgroup, 8- or 16-deep contraction blocks, and with its LDS regions unpadded;
+* the streaming schedule with 4 or 8 wavefronts, 4- or 8-row blocks and LDS read-ahead 0 or 2, plus
+  16-row blocks, the paired (interleaved) chunk layout of the shared operands in LDS (FP64) and
+  the coalesced row loads of the streamed operand;
+* with ``precision='mixed'`` also the matrix-core forms (FP32 MFMA and split BF16x3) of FP64
+  chains with dense shared operands; they change the result within 1e-5, so the default
+  ``'exact'`` space leaves them out.
+
+A candidate that the chain's shape does not admit (the emitter falls back to another form) is
+dropped, so every candidate label names the kernel that is actually built.
 """
 import itertools
 from dataclasses import asdict, dataclass
@@ -32,33 +41,79 @@ class Candidate:
     return asdict(self.schedule) if self.schedule is not None else {}
 
 
-def candidate_space():
+def candidate_space(precision='exact'):
   out = [Candidate('plain', 'plain', None)]
   for waves, kb in itertools.product((1, 4), (8, 16)):
     out.append(Candidate(f'cpl_w{waves}_k{kb}', 'tuned',
                          Schedule(stream=False, waves_per_block=waves, k_block=kb)))
+  out.append(Candidate('cpl_w4_k16_nopad', 'tuned',
+                       Schedule(stream=False, waves_per_block=4, k_block=16, lds_ld_pad=0,
+                                lds_elem_pad=0)))
   for waves, kb, la in itertools.product((4, 8), (4, 8), (0, 2)):
     out.append(Candidate(f'stream_w{waves}_k{kb}_a{la}', 'tuned',
                          Schedule(stream_waves=waves, stream_k_block=kb, stream_lds_lookahead=la)))
+  out.append(Candidate('stream_w8_k16_a2', 'tuned',
+                       Schedule(stream_waves=8, stream_k_block=16, stream_lds_lookahead=2)))
+  out.append(Candidate('stream_w8_k8_a2_pair', 'tuned',
+                       Schedule(stream_waves=8, stream_k_block=8, stream_lds_lookahead=2,
+                                stream_pair_reads=True)))
+  out.append(Candidate('stream_w8_k8_a2_rows', 'tuned',
+                       Schedule(stream_waves=8, stream_k_block=8, stream_lds_lookahead=2,
+                                stream_row_loads=True)))
+  if precision == 'mixed':
+    out.append(Candidate('mfma_fp32', 'mfma_fp32', None))
+    out.append(Candidate('mfma_bf16x3', 'mfma_bf16x3', None))
   return out
 
 
-def applicable(descrs, fp_type, cands=None):
-  """[(candidate, generated Generator)] for the candidates the chain admits."""
+def _kind(label):
+  return label.split('_')[0]
+
+
+def _admitted(c, em):
+  """The emitted kernel is the form the candidate names (no fallback to another form)."""
+  if c.emitter == 'plain':
+    return em.emitter == 'plain'
+  if c.emitter.startswith('mfma'):
+    return em.emitter == 'mfma'
+  kind = em.schedule.get('kind', 'column') if em.emitter == 'tuned' else em.emitter
+  want = 'stream' if _kind(c.label) == 'stream' else 'column'
+  if kind != want:
+    return False
+  sch = c.schedule
+  if want == 'stream':
+    # options the chain does not take are silently off in the emitter: not a distinct kernel
+    if sch.stream_pair_reads and em.schedule.get('pair') is not True:
+      return False
+    if sch.stream_row_loads and em.schedule.get('row_loads') is not True:
+      return False
+    if em.schedule.get('k_block') != sch.stream_k_block and not em.schedule.get('loop'):
+      return False
+  return True
+
+
+def applicable(descrs, fp_type, cands=None, name_prefix=None, precision='exact'):
+  """[(candidate, generated Generator)] for the candidates the chain admits. With
+  ``name_prefix`` every candidate's kernel and launcher get their own name
+  (``<prefix>_<label>``), so all of them can go into one shared object."""
   ctx = Context('gfx950', 'hip', fp_type)
   out = []
-  for c in cands or candidate_space():
+  seen = set()
+  for c in cands or candidate_space(precision):
     g = Generator(descrs, ctx, emitter=c.emitter, schedule=c.schedule)
+    if name_prefix is not None:
+      g.set_kernel_name(f'{name_prefix}_{c.label}')
     try:
       g.generate()
     except Exception:   # shape not admitted by this emitter
       continue
-    em = g.get_emitted()
-    kind = em.schedule.get('kind', 'column') if em.emitter == 'tuned' else 'plain'
-    want = 'plain' if c.emitter == 'plain' else ('stream' if c.label.startswith('stream') else
-                                                  'column')
-    if kind != want:
+    if not _admitted(c, g.get_emitted()):
       continue
+    # knobs the chain's kernel does not depend on give the same kernel twice: keep the first
+    body = g.get_kernel().replace(g.get_base_name(), '@')
+    if body in seen:
+      continue
+    seen.add(body)
     out.append((c, g))
   return out
 

@@ -4,11 +4,21 @@ database the cost model is trained on, and the yardstick for its picks.
     python -m tensorforge_amd.tune.sweep build   --dir D [--chains N] [--seed S]   (CPU: hipcc)
     python -m tensorforge_amd.tune.sweep measure --dir D                           (GPU)
 
-``build`` keeps the chains that admit at least two candidates, compiles every candidate into
-its own shared object (the candidates of one chain share the launcher name) and writes
-``D/index.json`` with each candidate's features. ``measure`` times every candidate on synthetic
-inputs at the chain's batch size (HIP events, median of three runs of ``steps`` launches) and
-appends one JSON line per candidate to ``D/timings.jsonl``.
+``build`` draws ``N`` random chains and keeps the ones that admit at least two candidates. All
+candidates of one chain go into one shared object under their own names
+(``launcher_c<chain>_<label>``), so one hipcc call builds a chain's whole candidate set. A plain
+or column-per-lane candidate whose generated source exceeds ``--max-source`` bytes is left out
+(its unrolled C++ takes from half a minute to minutes to compile); the streaming kernels (inline
+assembly, seconds to compile at any size) are always kept. Chains are
+generated and compiled in worker processes; ``D/index.json`` records each chain's kept
+candidates, features and launcher parameter list.
+
+``measure`` needs no kernel generation: it rebuilds each chain's descriptions (only to name the
+launcher parameters and size the synthetic inputs), loads the chain's object, and times every
+candidate on synthetic inputs at the chain's batch size: one untimed launch, then the median of
+per-launch HIP-event durations over up to ``steps`` launches (fewer for slow kernels). One JSON
+line per candidate is appended to ``D/timings.jsonl``; a chain already present there is skipped,
+so a run can be resumed.
 """
 import argparse
 import concurrent.futures
@@ -20,75 +30,155 @@ import time
 from . import features as F
 from . import space
 
-
-def _chains(count, seed):
-  return space.random_chains(count, seed)
+MAX_SOURCE = 500_000
 
 
-def build(directory, count, seed, jobs=8, min_candidates=2):
+def _chain_job(args):
+  """Worker: generate chain ``i``'s candidates and compile them into ``lib c<i>.so``."""
+  i, seed_chain, directory, max_source, precision = args
+  import numpy as np
   from ..runtime import build as B
-  os.makedirs(directory, exist_ok=True)
-  entries, work = [], []
-  for i, (descrs, fp, batch, info) in enumerate(_chains(count, seed)):
-    cands = space.applicable(descrs, fp)
-    if len(cands) < min_candidates:
-      continue
-    rec = {'chain': i, 'info': info, 'batch': batch, 'candidates': []}
-    for j, (c, g) in enumerate(cands):
-      name = f'c{i}_{j}'
-      rec['candidates'].append({'label': c.label, 'so': f'lib{name}.so',
-                                'features': F.features(g, c, batch, info)})
-      work.append((g, name))
-    entries.append(rec)
-
-  def one(item):
-    g, name = item
-    return B.compile_generators([g], name, directory)
-
+  rng = np.random.default_rng([seed_chain, i])
+  chain = None
+  while chain is None:
+    chain = space.random_chain(rng)
+  descrs, fp, batch, info = chain
   t0 = time.time()
-  with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
-    for k, _ in enumerate(pool.map(one, work)):
-      if k % 25 == 0:
-        print(f'compiled {k + 1}/{len(work)} ({time.time() - t0:.0f} s)', flush=True)
-  for f in os.listdir(directory):
-    if f.endswith('.hip'):
-      os.remove(os.path.join(directory, f))
-  with open(os.path.join(directory, 'index.json'), 'w') as f:
-    json.dump({'count': count, 'seed': seed, 'feature_names': F.NAMES, 'chains': entries}, f)
-  print(f'{len(entries)} chains, {len(work)} kernels', flush=True)
+  cands = space.applicable(descrs, fp, name_prefix=f'c{i}', precision=precision)
+  kept, dropped = [], []
+  for c, g in cands:
+    # streaming kernels are straight-line inline assembly and compile in seconds at any size;
+    # the plain and column-per-lane kernels' unrolled C++ takes minutes above the cap
+    big = not c.label.startswith('stream') and len(g.get_kernel()) > max_source
+    (dropped if big else kept).append((c, g))
+  rec = {'chain': i, 'info': info, 'batch': batch, 'dropped_large': [c.label for c, _ in dropped],
+         'candidates': []}
+  if len(kept) < 2:
+    return rec, time.time() - t0, 0.0
+  for c, g in kept:
+    rec['candidates'].append({'label': c.label, 'launcher': f'launcher_{g.get_base_name()}',
+                              'features': F.features(g, c, batch, info)})
+  t1 = time.time()
+  B.compile_generators([g for _, g in kept], f'c{i}', directory)
+  src = os.path.join(directory, f'c{i}.hip')
+  if os.path.exists(src):
+    os.remove(src)
+  rec['so'] = f'libc{i}.so'
+  return rec, t1 - t0, time.time() - t1
 
 
-def measure(directory, steps=5, out_path=None):
+def chain_descriptions(seed_chain, i):
+  """The descriptions of random chain ``i`` (same draw as the build)."""
+  import numpy as np
+  rng = np.random.default_rng([seed_chain, i])
+  chain = None
+  while chain is None:
+    chain = space.random_chain(rng)
+  return chain
+
+
+def build(directory, count, seed, jobs=8, max_source=MAX_SOURCE, precision='exact', start=0):
+  os.makedirs(directory, exist_ok=True)
+  index_path = os.path.join(directory, 'index.json')
+  entries, skipped, gen_s, cc_s = [], 0, 0.0, 0.0
+  if os.path.exists(index_path):
+    old = json.load(open(index_path))
+    entries, skipped = old['chains'], old.get('skipped', 0)
+    gen_s, cc_s = old.get('generate_seconds', 0.0), old.get('compile_seconds', 0.0)
+  done = {e['chain'] for e in entries}
+  todo = [(i, seed, directory, max_source, precision) for i in range(start, start + count)
+          if i not in done]
+  t0 = time.time()
+
+  def save():
+    with open(index_path + '.tmp', 'w') as f:
+      json.dump({'seed': seed, 'feature_names': F.NAMES, 'max_source': max_source,
+                 'precision': precision, 'chains': sorted(entries, key=lambda e: e['chain']),
+                 'skipped': skipped, 'generate_seconds': gen_s, 'compile_seconds': cc_s}, f)
+    os.replace(index_path + '.tmp', index_path)
+
+  with concurrent.futures.ProcessPoolExecutor(jobs) as pool:
+    for k, (rec, g_s, c_s) in enumerate(pool.map(_chain_job, todo, chunksize=1)):
+      gen_s += g_s
+      cc_s += c_s
+      if rec['candidates']:
+        entries.append(rec)
+      else:
+        skipped += 1
+      if k % 20 == 19 or k + 1 == len(todo):
+        save()
+        print(f'{k + 1}/{len(todo)} chains drawn, {len(entries)} kept '
+              f'({time.time() - t0:.0f} s)', flush=True)
+  save()
+
+
+def _time_candidate(fn, args, steps, budget_s=0.02):
+  """Median per-launch duration (s): one untimed launch, then up to ``steps`` timed ones (fewer
+  when one launch takes more than ``budget_s / steps``)."""
   import torch
-  from ..runtime.loader import load_kernels
-  from .measure import synthetic_buffers, time_launcher
+  stream = torch.cuda.current_stream()
+  e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+  e0.record(stream)
+  fn(*args)
+  e1.record(stream)
+  torch.cuda.synchronize()
+  first = e0.elapsed_time(e1) / 1e3
+  n = int(min(steps, max(2, budget_s / max(first, 1e-7))))
+  ev = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
+  ev[0].record(stream)
+  for j in range(n):
+    fn(*args)
+    ev[j + 1].record(stream)
+  torch.cuda.synchronize()
+  ts = sorted(ev[j].elapsed_time(ev[j + 1]) / 1e3 for j in range(n))
+  return ts[len(ts) // 2] if n % 2 else 0.5 * (ts[n // 2 - 1] + ts[n // 2])
+
+
+def measure(directory, steps=7, out_path=None, limit=None):
+  import ctypes
+  import torch
+  from ..common.context import Context
+  from ..device import launch_args
+  from ..generators.generator import Generator
+  from ..runtime.loader import Kernel, runtime
+  from .measure import synthetic_buffers
   idx = json.load(open(os.path.join(directory, 'index.json')))
-  chains = _chains(idx['count'], idx['seed'])
   dev = torch.device('cuda:0')
   out_path = out_path or os.path.join(directory, 'timings.jsonl')
-  with open(out_path, 'w') as out:
-    for n, rec in enumerate(idx['chains']):
-      descrs, fp, batch, info = chains[rec['chain']]
-      cands = dict((c.label, (c, g)) for c, g in space.applicable(descrs, fp))
-      bufs = None
+  have = set()
+  if os.path.exists(out_path):
+    with open(out_path) as f:
+      have = {json.loads(line)['chain'] for line in f if line.strip()}
+  runtime()
+  chains = [r for r in idx['chains'] if r['chain'] not in have]
+  if limit is not None:
+    chains = chains[:limit]
+  t0 = time.time()
+  with open(out_path, 'a') as out:
+    for n, rec in enumerate(chains):
+      descrs, fp, batch, info = chain_descriptions(idx['seed'], rec['chain'])
+      g = Generator(descrs, Context('gfx950', 'hip', fp))
+      g.register()
+      lib = ctypes.CDLL(os.path.join(directory, rec['so']))
+      bufs = synthetic_buffers(g, batch, dev)
+      args = launch_args(g, {k: v for k, v in bufs.items() if k != '__keep__'}, batch)
+      lines = []
       for cand in rec['candidates']:
-        c, g = cands[cand['label']]
-        k = load_kernels(os.path.join(directory, cand['so']), [g])[0]
-        if bufs is None:
-          bufs = synthetic_buffers(g, batch, dev)
+        g.set_kernel_name(cand['launcher'][len('launcher_'):])
+        k = Kernel(g, lib)
         try:
-          first = time_launcher(k, bufs, batch, steps=1, warmup=1)
+          t = _time_candidate(k, args, steps)
         except RuntimeError as err:   # the launcher reported a launch error: not a candidate
           print(f'chain {rec["chain"]} {cand["label"]}: {err}', flush=True)
           continue
-        n_steps = int(min(steps, max(1, 0.02 / max(first, 1e-7))))
-        ts = sorted(time_launcher(k, bufs, batch, steps=n_steps, warmup=0) for _ in range(3))
-        out.write(json.dumps({'chain': rec['chain'], 'label': cand['label'], 'time': ts[1],
-                              'batch': batch, 'info': info, 'features': cand['features']}) + '\n')
-      del bufs
+        lines.append(json.dumps({'chain': rec['chain'], 'label': cand['label'], 'time': t,
+                                 'batch': batch, 'info': info, 'features': cand['features']}))
+      out.write(''.join(line + '\n' for line in lines))
+      out.flush()
+      del bufs, args
       torch.cuda.empty_cache()
-      print(f'chain {n + 1}/{len(idx["chains"])} ({info["kind"]}, {len(rec["candidates"])} '
-            f'candidates) done', flush=True)
+      if n % 25 == 24 or n + 1 == len(chains):
+        print(f'{n + 1}/{len(chains)} chains timed ({time.time() - t0:.0f} s)', flush=True)
 
 
 def main(argv=None):
@@ -96,14 +186,18 @@ def main(argv=None):
   ap.add_argument('phase', choices=['build', 'measure'])
   ap.add_argument('--dir', required=True)
   ap.add_argument('--chains', type=int, default=150)
+  ap.add_argument('--start', type=int, default=0, help='index of the first chain drawn')
   ap.add_argument('--seed', type=int, default=5)
   ap.add_argument('--jobs', type=int, default=8)
+  ap.add_argument('--max-source', type=int, default=MAX_SOURCE)
+  ap.add_argument('--precision', choices=['exact', 'mixed'], default='exact')
+  ap.add_argument('--limit', type=int, default=None, help='time at most this many chains')
   ap.add_argument('--out', default=None, help='timings file (default: DIR/timings.jsonl)')
   a = ap.parse_args(argv)
   if a.phase == 'build':
-    build(a.dir, a.chains, a.seed, a.jobs)
+    build(a.dir, a.chains, a.seed, a.jobs, a.max_source, a.precision, a.start)
   else:
-    measure(a.dir, out_path=a.out)
+    measure(a.dir, out_path=a.out, limit=a.limit)
 
 
 if __name__ == '__main__':

@@ -77,3 +77,35 @@ def test_ensemble_predicts_the_mean_of_its_networks():
           for lab, t in (('plain', 3.0), ('tuned/a', 1.0), ('tuned/b', 2.0))]
   s = M.evaluate(ens, recs, prune_k=1)
   assert s['model_pick']['mean_slowdown'] == 0.0
+
+
+def test_named_candidates_are_distinct_kernels_of_one_chain():
+  """With a name prefix every candidate gets its own kernel and launcher name (one shared object
+  holds them all), and knobs the chain's kernel does not depend on give no duplicate."""
+  w = W.ader_volume()
+  cands = space.applicable(w.descrs, w.fp_type, name_prefix='c7')
+  names = [g.get_base_name() for _, g in cands]
+  assert len(set(names)) == len(names)
+  assert all(n == f'c7_{c.label}' for (c, _), n in zip(cands, names))
+  bodies = [g.get_kernel().replace(g.get_base_name(), '@') for _, g in cands]
+  assert len(set(bodies)) == len(bodies)
+  labels = [c.label for c, _ in cands]
+  # the paired LDS layout of the shared operand applies to this FP64 chain
+  assert 'stream_w8_k8_a2_pair' in labels
+
+
+def test_matrix_core_candidates_only_in_the_mixed_precision_space():
+  w = W.ader_volume(b=84, sparse_k=False)
+  exact = [c.label for c, _ in space.applicable(w.descrs, w.fp_type)]
+  mixed = [c.label for c, _ in space.applicable(w.descrs, w.fp_type, precision='mixed')]
+  assert not any(l.startswith('mfma') for l in exact)
+  assert {'mfma_fp32', 'mfma_bf16x3'} <= set(mixed)
+
+
+def test_sweep_chain_draws_are_reproducible_per_index():
+  from tensorforge_amd.tune import sweep
+  a = sweep.chain_descriptions(5, 123)
+  b = sweep.chain_descriptions(5, 123)
+  c = sweep.chain_descriptions(5, 124)
+  assert a[3] == b[3] and a[2] == b[2]
+  assert (a[3], a[2]) != (c[3], c[2])
