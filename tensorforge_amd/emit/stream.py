@@ -44,6 +44,9 @@ class Term:
   B: Optional[ir.Factor] = None     # per-element right operand of a 'global' term
   b_dims: tuple = ()
   entries: list = field(default_factory=list)   # (row rel, k, position in U)
+  L: Optional[ir.Factor] = None     # streamed operand of a fused term, and its column window
+  l_cols: tuple = ()
+  l_shift: int = 0                  # row of L (temporary coordinates) that contraction index 0 reads
 
 
 @dataclass
@@ -65,6 +68,10 @@ class Plan:
   post: list = field(default_factory=list)
   final_init: str = 'zero'
   final_beta: float = 0.0
+  # fused terms stream different operands (the time-derivative step that is not aligned): their
+  # contraction blocks run one term after the other, each block's rows loaded from its term's
+  # operand into the same ring
+  stacked: bool = False
 
 
 @dataclass
@@ -222,17 +229,19 @@ def _plan_single(program):
       if id(R.tensor) not in by_tmp or sh.right_dims != (0, 1):
         return None, 'temporary not formed by a streamable product'
       prod, psh = by_tmp.pop(id(R.tensor))
-      if prod.n_ranges[0][0] != 0 or prod.n_ranges[0][1] < kr[1] or kr[0] != 0:
+      shift = prod.n_ranges[0][0]
+      if prod.n_ranges[0][1] - shift < kr[1] or kr[0] != 0:
         return None, 'temporary rows do not match the contraction'
       if prod.n_ranges[1][0] != 0 or prod.n_ranges[1][1] < q:
         return None, 'temporary columns do not match'
       if L is None:
         L = psh.left
         l_window = tuple(prod.k_ranges[0])
-      elif psh.left.arg is not L.arg or tuple(prod.k_ranges[0]) != l_window:
-        return None, 'temporaries stream different operands'
+      elif tuple(prod.k_ranges[0]) != l_window:
+        return None, 'temporaries stream different column windows'
       terms.append(Term(p, U, sh.left_dims, 'fused', producer=prod, W=psh.right,
-                        w_dims=psh.right_dims))
+                        w_dims=psh.right_dims, L=psh.left, l_cols=tuple(prod.k_ranges[0]),
+                        l_shift=shift))
     elif R.src == ir.SRC_GLOBAL and R.arg.per_element and not R.is_sparse():
       if any(tuple(t.product.k_ranges[0]) != kr for t in terms):
         return None, 'products with different contraction ranges'
@@ -241,9 +250,20 @@ def _plan_single(program):
       return None, 'right operand kind'
   if by_tmp:
     return None, 'temporary used other than by a single streamed product'
+  stacked = len({id(t.L.arg) for t in terms if t.kind == 'fused'}) > 1
+  if not stacked and any(t.l_shift for t in terms if t.kind == 'fused'):
+    return None, 'temporary rows start inside the streamed operand'
+  if stacked:
+    if any(t.kind != 'fused' for t in terms):
+      return None, 'streamed operands of several temporaries beside a per-element operand'
+    for t in terms:
+      if t.L.arg.addressing != Addressing.STRIDED or t.L.strides != L.strides or \
+          t.L.origin[1] != L.origin[1]:
+        return None, 'temporaries stream operands of different layouts'
   # every product must be a consumer or its producer
   return Plan(dest=dest, dest_product=consumers[0], rows=rows, cols=cols, q=q, klo=klo, khi=khi,
-              init=init, beta=beta, terms=terms, L=L, l_cols=l_window or ()), None
+              init=init, beta=beta, terms=terms, L=L, l_cols=l_window or (),
+              stacked=stacked), None
 
 
 class _Stream:
@@ -293,14 +313,21 @@ class _Stream:
         i, k = idx[nd], idx[kd]
         if pl.rows[0] <= i < pl.rows[1] and max(pl.klo, tk0) <= k < min(pl.khi, tk1):
           t.entries.append((i - lo0, k, pos))
-    blocks = list(range(pl.klo, pl.khi, self.KB))
-    for b0 in blocks:
+    if pl.stacked:
+      # one term after the other, each over its own contraction range
+      blocks = []
+      for t_idx, t in enumerate(pl.terms):
+        tk0, tk1 = t.product.k_ranges[0]
+        blocks += [(b0, min(b0 + self.KB, tk1), (t_idx,)) for b0 in range(tk0, tk1, self.KB)]
+    else:
+      blocks = [(b0, min(b0 + self.KB, pl.khi), None) for b0 in range(pl.klo, pl.khi, self.KB)]
+    for bi, (b0, b1, only) in enumerate(blocks):
       # within a block: contraction index, then term, then row, so the right operand's row k
       # of every term is consumed right after it is formed
       es = [(e[1], t_idx, e[0], e) for t_idx, t in enumerate(pl.terms) for e in t.entries
-            if b0 <= e[1] < b0 + self.KB]
+            if (only is None or t_idx in only) and b0 <= e[1] < b1]
       for _, t_idx, _, e in sorted(es, key=lambda x: x[:3]):
-        seq.append((b0, t_idx, e))
+        seq.append((bi, t_idx, e))
     self.blocks = blocks
     self.seq = seq
     # register stages: entries by previous-stage row, then row, after the streamed entries
@@ -326,7 +353,7 @@ class _Stream:
     """Dense shared operands with many entries run the contraction blocks as a runtime loop
     (the fully unrolled body would not fit the instruction cache)."""
     pl = self.pl
-    if self.sched.stream_loop == 'never' or pl.post:
+    if self.sched.stream_loop == 'never' or pl.post or pl.stacked:
       return False
     if any(t.U.is_sparse() for t in pl.terms):
       return False
@@ -343,6 +370,12 @@ class _Stream:
     pl = self.pl
     self._g0_declared = False
     self.order_entries()
+    # streamed operands: one, or one per term when the terms stream different operands
+    self.Ls, self.L_index = [], {}
+    for L in ([t.L for t in pl.terms] if pl.stacked else [pl.L]):
+      if L is not None and id(L.arg) not in self.L_index:
+        self.L_index[id(L.arg)] = len(self.Ls)
+        self.Ls.append(L)
     self.loop = self._loop_mode()
     if self.loop and self.KB != self.sched.stream_loop_k_block:
       # the runtime block loop keeps one block of every term in flight: smaller blocks
@@ -350,7 +383,8 @@ class _Stream:
       self.order_entries()
       self.loop = self._loop_mode()
     # shared entries read two chunks per instruction, streamed rows two per instruction
-    self.pair = bool(self.sched.stream_pair_reads) and not self.loop and self.real == 'double'
+    self.pair = bool(self.sched.stream_pair_reads) and not self.loop and self.real == 'double' \
+        and not pl.stacked
     if self.pair:
       self.nu_pad = (self.nu + 2 * CHUNK - 1) // (2 * CHUNK) * (2 * CHUNK)
     self.rows_pad = (pl.rows[1] - pl.rows[0] + CHUNK - 1) // CHUNK * CHUNK
@@ -370,6 +404,9 @@ class _Stream:
     self.dma = False
     if pl.L is not None:
       self.l_q = pl.l_cols[1] - pl.l_cols[0]
+      if pl.stacked and not self.own_columns():
+        raise GenerationError('operands streamed one term after the other need the lanes to '
+                              'load their own columns')
       self.pairs = G * self.l_q
       self.ps = KB + 2 if (self.pair and KB % 2 == 0) else KB + 1
       self.slot_words = self.pairs * self.ps
@@ -616,12 +653,13 @@ class _Stream:
       return
     D = self.D
     self._nxt_ptr_done = False
+    self._nxt_ptrs = set()
     if self.own_columns():
-      w(f'const {pl.L.arg.dtype}* __restrict__ lp_cur = {self.lane_pointer("e0")};')
+      self.declare_lane_pointers('e0', 'cur')
     if self.row_loads():
       self.row_offsets('e0', 'ro')
-    for bi, b0 in enumerate(self.blocks):
-      w(f'// block [{b0}, {min(b0 + self.KB, pl.khi)})')
+    for bi, (b0, b1, only) in enumerate(self.blocks):
+      w(f'// block [{b0}, {b1})' + (f' of term {only[0]}' if only else ''))
       # issue the loads of the block D ahead (this group's, or the next group's)
       tb = bi + D
       if tb < nb:
@@ -783,8 +821,9 @@ class _Stream:
     w._depth += 1
     w('const size_t e0 = efirst;')
     w('const size_t e0n = efirst + estep;')
+    self._nxt_ptrs = set()
     if self.own_columns():
-      w(f'const {pl.L.arg.dtype}* __restrict__ lp_cur = {self.lane_pointer("e0")};')
+      self.declare_lane_pointers('e0', 'cur')
     for j in range(D):
       if j < len(self.blocks):
         self.pipeline_load(j, 'e0', j, declare=False)
@@ -801,7 +840,7 @@ class _Stream:
     alignment is checked by the launcher)."""
     pl = self.pl
     L = pl.L
-    if not (self.sched.stream_dma and not self.loop and self.own_columns()):
+    if not (self.sched.stream_dma and not self.loop and self.own_columns()) or self.pl.stacked:
       return False
     if any(t.kind == 'global' for t in pl.terms) or len(self.blocks) < 2:
       return False
@@ -867,8 +906,8 @@ class _Stream:
     w(f'const unsigned gs = (unsigned)((r * {nb}) % {NS});')
     # at most the loads of the D - 1 younger blocks may still be in flight
     pending = min(63, (D - 1) * (KB // self.rpl))
-    for bi, b0 in enumerate(self.blocks):
-      w(f'// block [{b0}, {min(b0 + KB, pl.khi)})')
+    for bi, (b0, b1, _) in enumerate(self.blocks):
+      w(f'// block [{b0}, {b1})')
       w(f'asm volatile("s_waitcnt vmcnt({pending})" ::: "memory");')
       w('{')
       w._depth += 1
@@ -902,7 +941,7 @@ class _Stream:
     per element group. Needs full blocks."""
     pl = self.pl
     return (pl.L is not None and pl.L.arg.addressing == Addressing.STRIDED and
-            self.sched.stream_row_loads and not self.loop and
+            self.sched.stream_row_loads and not self.loop and not pl.stacked and
             (pl.khi - pl.klo) % self.KB == 0 and
             not any(t.kind == 'global' for t in pl.terms))
 
@@ -931,13 +970,23 @@ class _Stream:
         w(f'{name}{s_} = ev * {a.element_stride} + (cc + {pl.l_cols[0] - L.origin[1]}) * '
           f'{L.strides[1]} + (ix - pr * {KB} + {pl.klo - L.origin[0]}) * {L.strides[0]};')
 
-  def lane_pointer(self, ebase):
+  def lane_pointer(self, ebase, L=None):
     """Expression of the lane's pointer into its element's column of L for group ``ebase``."""
-    L = self.pl.L
+    L = L or self.pl.L
     a = L.arg
     e = f'({ebase} + slot < numElements ? {ebase} + slot : 0)'
     return (f'({a.name} + {e} * {a.element_stride}ull + {a.name}_extraOffset + '
             f'(col + {self.pl.l_cols[0] - L.origin[1]}) * {L.strides[1]})')
+
+  def lp_name(self, which, li):
+    """Name of the lane's pointer into streamed operand ``li`` ('cur' or 'nxt' group)."""
+    return f'lp_{which}' + (str(li) if li else '')
+
+  def declare_lane_pointers(self, ebase, which):
+    for li, L in enumerate(self.Ls):
+      w_ = self.w
+      w_(f'const {L.arg.dtype}* __restrict__ {self.lp_name(which, li)} = '
+         f'{self.lane_pointer(ebase, L)};')
 
   def pipeline_load(self, bi, ebase, pos, declare=False):
     """Registers of logical pipeline position ``pos`` <- block ``bi`` of the group at
@@ -948,13 +997,15 @@ class _Stream:
     b0, b1 = self._block_rows(bi)
     nr = b1 - b0
     if pl.L is not None and self.own_columns():
-      L = pl.L
-      ptr = 'lp_cur' if ebase == 'e0' else 'lp_nxt'
-      if ebase != 'e0' and not getattr(self, '_nxt_ptr_done', False):
-        w(f'const {L.arg.dtype}* __restrict__ lp_nxt = {self.lane_pointer("e0n")};')
-        self._nxt_ptr_done = True
+      li, L = self._block_L(bi)
+      only = self.blocks[bi][2]
+      shift = pl.terms[only[0]].l_shift if only else 0
+      ptr = self.lp_name('cur' if ebase == 'e0' else 'nxt', li)
+      if ebase != 'e0' and li not in self._nxt_ptrs:
+        w(f'const {L.arg.dtype}* __restrict__ {ptr} = {self.lane_pointer("e0n", L)};')
+        self._nxt_ptrs.add(li)
       for kk in range(nr):
-        w(f'pf{reg}[{kk}] = ({self.real}){ptr}[{(b0 + kk - L.origin[0]) * L.strides[0]}];')
+        w(f'pf{reg}[{kk}] = ({self.real}){ptr}[{(b0 + kk + shift - L.origin[0]) * L.strides[0]}];')
     elif pl.L is not None and self.row_loads():
       L = pl.L
       if ebase == 'e0':
@@ -1221,8 +1272,16 @@ class _Stream:
 
   # ---- ring of the streamed operand ----
   def _block_rows(self, bi):
-    b0 = self.blocks[bi]
-    return b0, min(b0 + self.KB, self.pl.khi)
+    b0, b1, _ = self.blocks[bi]
+    return b0, b1
+
+  def _block_L(self, bi):
+    """(index into self.Ls, streamed operand) of block bi's rows."""
+    only = self.blocks[bi][2]
+    if not self.pl.stacked:
+      return 0, self.pl.L
+    L = self.pl.terms[only[0]].L
+    return self.L_index[id(L.arg)], L
 
   def load_block(self, bi, stage):
     """Load the block's rows of L for every (element, column) pair of this wavefront into
@@ -1303,10 +1362,11 @@ class _Stream:
     w = self.w
     pl = self.pl
     real = self.real
-    b1 = min(b0 + self.KB, pl.khi)
+    b1 = self.blocks[bi][1]
     ring_slot = bi % 2
-    block_seq = [(n, s) for n, s in enumerate(self.seq) if s[0] == b0]
-    fused_terms = [ti for ti, t in enumerate(pl.terms) if t.kind == 'fused']
+    block_seq = [(n, s) for n, s in enumerate(self.seq) if s[0] == bi]
+    fused_terms = [ti for ti, t in enumerate(pl.terms) if t.kind == 'fused' and
+                   (self.blocks[bi][2] is None or ti in self.blocks[bi][2])]
     offs = sorted(set(o for ti in fused_terms for _, o in self.slots[ti]))
     # sequence point: the block's LDS reads use offsets rewritten here, so the compiler cannot
     # hoist them above the previous block's FMAs (which would keep every operand live at once)
@@ -1357,11 +1417,12 @@ class _Stream:
     w = self.w
     pl = self.pl
     real = self.real
-    b1 = min(b0 + self.KB, pl.khi)
+    b1 = self.blocks[bi][1]
     ring_slot = bi % 2
     ahead = self.sched.stream_lds_lookahead
-    block_seq = [(n, s) for n, s in enumerate(self.seq) if s[0] == b0]
-    fused_terms = [ti for ti, t in enumerate(pl.terms) if t.kind == 'fused']
+    block_seq = [(n, s) for n, s in enumerate(self.seq) if s[0] == bi]
+    fused_terms = [ti for ti, t in enumerate(pl.terms) if t.kind == 'fused' and
+                   (self.blocks[bi][2] is None or ti in self.blocks[bi][2])]
     offs = sorted(set(o for ti in fused_terms for _, o in self.slots[ti]))
     w(f'asm volatile("" : {", ".join(f"{chr(34)}+v{chr(34)}({o})" for o in ["lane15"] + offs)});')
     steps = []

@@ -68,7 +68,8 @@ def source_digest(source: str) -> str:
   return hashlib.sha1(source.encode()).hexdigest()[:12]
 
 
-def compile_source(source: str, name: str, out_dir: str = LIB_DIR, keep_source=True) -> str:
+def compile_source(source: str, name: str, out_dir: str = LIB_DIR, keep_source=True,
+                   extra_flags=()) -> str:
   """Compile one HIP translation unit into ``<out_dir>/lib<name>.so`` linked against the
   runtime library (found next to it at load time through ``$ORIGIN``)."""
   os.makedirs(out_dir, exist_ok=True)
@@ -77,21 +78,22 @@ def compile_source(source: str, name: str, out_dir: str = LIB_DIR, keep_source=T
     f.write(source)
   out = os.path.join(out_dir, f'lib{name}.so')
   rt_dir = os.path.dirname(runtime_path())
-  cmd = [hipcc()] + hipcc_flags() + [f'-I{INCLUDE_DIR}', src_path, f'-L{rt_dir}',
-                                     '-ltensorforge_rt', f'-Wl,-rpath,{rt_dir}',
-                                     '-Wl,-rpath,$ORIGIN']
+  cmd = [hipcc()] + hipcc_flags() + list(extra_flags) + [
+    f'-I{INCLUDE_DIR}', src_path, f'-L{rt_dir}', '-ltensorforge_rt', f'-Wl,-rpath,{rt_dir}',
+    '-Wl,-rpath,$ORIGIN']
   _atomic_compile(cmd, out, f'kernel build of {name}')
   if not keep_source:
     os.remove(src_path)
   return out
 
 
-def compile_generators(generators, name: str, out_dir: str = LIB_DIR) -> str:
-  """Generate (if needed) and compile ``generators`` into one shared object."""
+def compile_generators(generators, name: str, out_dir: str = LIB_DIR, extra_flags=()) -> str:
+  """Generate (if needed) and compile ``generators`` into one shared object (``extra_flags``:
+  further hipcc options, e.g. ``--offload-compress`` for objects kept in bulk)."""
   from ..emit.launcher import translation_unit
   for g in generators:
     if g.get_kernel() is None:
       g.generate()
   if not os.path.exists(runtime_path()):
     build_runtime()
-  return compile_source(translation_unit(generators), name, out_dir)
+  return compile_source(translation_unit(generators), name, out_dir, extra_flags=extra_flags)

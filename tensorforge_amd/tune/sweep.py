@@ -44,7 +44,13 @@ def _chain_job(args):
     chain = space.random_chain(rng)
   descrs, fp, batch, info = chain
   t0 = time.time()
-  cands = space.applicable(descrs, fp, name_prefix=f'c{i}', precision=precision)
+  # the tuned forms first: a chain none of them admits has nothing to choose (only the plain
+  # kernel) and is skipped before the plain kernel is generated
+  space_ = space.candidate_space(precision)
+  tuned = space.applicable(descrs, fp, [c for c in space_ if c.emitter != 'plain'],
+                           name_prefix=f'c{i}')
+  cands = (space.applicable(descrs, fp, [c for c in space_ if c.emitter == 'plain'],
+                            name_prefix=f'c{i}') + tuned) if tuned else []
   kept, dropped = [], []
   for c, g in cands:
     # streaming kernels are straight-line inline assembly and compile in seconds at any size;
@@ -59,7 +65,9 @@ def _chain_job(args):
     rec['candidates'].append({'label': c.label, 'launcher': f'launcher_{g.get_base_name()}',
                               'features': F.features(g, c, batch, info)})
   t1 = time.time()
-  B.compile_generators([g for _, g in kept], f'c{i}', directory)
+  # device code stored compressed: thousands of objects travel to the GPU for timing
+  B.compile_generators([g for _, g in kept], f'c{i}', directory,
+                       extra_flags=['--offload-compress'])
   src = os.path.join(directory, f'c{i}.hip')
   if os.path.exists(src):
     os.remove(src)
