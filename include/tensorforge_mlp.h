@@ -24,6 +24,17 @@ int tensorforge_mlp_backward(const float* params, const float* X, const float* w
                              const float* out, const float* target, float* grads, float* dwork,
                              int B, int in, int hidden, void* stream);
 
+/* out[b] for X[B][in] like tensorforge_mlp_forward, without a workspace and without allocating:
+ * the hidden activations live in a device array of the library (one per GPU, held until the
+ * process ends). No counterpart in the reference; for callers that cannot allocate (code under
+ * HIP graph capture, a tuning loop inside a serving process). B * hidden must not exceed
+ * tensorforge_mlp_predict_max_activations() (262,144); larger batches return
+ * hipErrorInvalidValue without launching. Calls that overlap on one GPU share the array and
+ * race: serialise them. */
+long tensorforge_mlp_predict_max_activations(void);
+int tensorforge_mlp_predict(const float* params, const float* X, float* out, int B, int in,
+                            int hidden, void* stream);
+
 /* One Adam step (bias-corrected, step >= 1) over n parameters with moments m, v. */
 int tensorforge_mlp_adam(float* params, const float* grads, float* m, float* v, long n, float lr,
                          float beta1, float beta2, float eps, int step, void* stream);

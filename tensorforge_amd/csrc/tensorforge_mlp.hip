@@ -7,6 +7,7 @@
 // tensorforge/interop.py:20-23).
 #include <hip/hip_runtime.h>
 #include <cstddef>
+#include <mutex>
 
 extern "C" void tensorforge_record_error(int code, const char* where);
 
@@ -205,4 +206,75 @@ extern "C" int tensorforge_mlp_adam(float* params, const float* grads, float* m,
                      static_cast<hipStream_t>(stream), params, grads, m, v, n, lr, beta1, beta2,
                      eps, c1, c2);
   return finish("tensorforge_mlp_adam");
+}
+
+// ---------------------------------------------------------------------------------------------
+// Inference without allocation: the hidden activations live in a file-scope device array instead
+// of a caller workspace, for callers that cannot allocate (code under HIP graph capture, a tuning
+// loop inside a serving process). Each GPU the library runs on holds its own copy of the array
+// until the process ends. Every call on one GPU shares that GPU's array: calls that overlap on one
+// GPU (two streams, two threads) race and must be serialised by the caller. Nothing needs
+// clearing between calls: each layer overwrites all B * hidden activations it later reads.
+namespace {
+
+constexpr long kPredictMaxActivations = 1L << 18;   // B * hidden <= 262,144 (3 MiB of floats)
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) float scratch[3 * kPredictMaxActivations];
+
+std::once_flag g_scratch_once[kMaxDevices];
+float* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_status[kMaxDevices] = {};
+
+// The device address of this GPU's copy of `scratch`, looked up once per device with that device
+// current (the array's name in host code is only a host-side stand-in, never a device address).
+hipError_t scratch_on_current_device(float** ptr) {
+  int dev = 0;
+  hipError_t err = hipGetDevice(&dev);
+  if (err != hipSuccess) return err;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::call_once(g_scratch_once[dev], [dev]() {
+    void* p = nullptr;
+    g_scratch_status[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  });
+  if (g_scratch_status[dev] != hipSuccess) return g_scratch_status[dev];
+  *ptr = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" long tensorforge_mlp_predict_max_activations(void) { return kPredictMaxActivations; }
+
+extern "C" int tensorforge_mlp_predict(const float* params, const float* X, float* out, int B,
+                                       int in, int hidden, void* stream) {
+  if (B <= 0 || in <= 0 || hidden <= 0) return static_cast<int>(hipErrorInvalidValue);
+  const long bh = static_cast<long>(B) * hidden;
+  if (bh > kPredictMaxActivations) {
+    tensorforge_record_error(static_cast<int>(hipErrorInvalidValue),
+                             "tensorforge_mlp_predict: B * hidden above "
+                             "tensorforge_mlp_predict_max_activations()");
+    return static_cast<int>(hipErrorInvalidValue);
+  }
+  float* work = nullptr;
+  const hipError_t err = scratch_on_current_device(&work);
+  if (err != hipSuccess) {
+    tensorforge_record_error(static_cast<int>(err), "tensorforge_mlp_predict: scratch address");
+    return static_cast<int>(err);
+  }
+  const Layout l = layout(in, hidden);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  float* a1 = work;
+  float* a2 = work + bh;
+  float* a3 = work + 2 * bh;
+  hipLaunchKernelGGL(dense_forward<true>, dim3(grid_for(bh)), dim3(kThreads), 0, s,
+                     params + l.w1, params + l.b1, X, a1, B, in, hidden);
+  hipLaunchKernelGGL(dense_forward<true>, dim3(grid_for(bh)), dim3(kThreads), 0, s,
+                     params + l.w2, params + l.b2, a1, a2, B, hidden, hidden);
+  hipLaunchKernelGGL(dense_forward<true>, dim3(grid_for(bh)), dim3(kThreads), 0, s,
+                     params + l.w3, params + l.b3, a2, a3, B, hidden, hidden);
+  hipLaunchKernelGGL(dense_forward<false>, dim3(grid_for(B)), dim3(kThreads), 0, s,
+                     params + l.w4, params + l.b4, a3, out, B, hidden, 1);
+  return finish("tensorforge_mlp_predict");
 }
