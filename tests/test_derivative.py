@@ -60,19 +60,31 @@ def test_streaming_plan_keeps_each_term_in_its_range():
               emitter='mfma_fp32').generate()
 
 
-def test_three_streamed_arrays_fall_back_to_the_plain_kernel():
-  """Three streamed arrays are outside the streaming form; the column-per-lane form needs
-  about 82 KB of LDS per one-wavefront workgroup (one wavefront per CU), so the default choice
-  is the plain kernel; asking for the tuned emitter still gives the column-per-lane form."""
-  from tensorforge_amd.emit import tuned
+def test_three_streamed_arrays_stream_one_term_after_the_other():
+  """Three streamed arrays (the step that is not aligned): the streaming form runs each term's
+  contraction blocks in turn, loading each block's rows from its own array (row 1 of the array
+  is contraction index 0), so the default choice is the streaming kernel. Without the streaming
+  form the column-per-lane form needs about 82 KB of LDS per one-wavefront workgroup (one
+  wavefront per CU) and counts as starved."""
+  from tensorforge_amd.emit import stream, tuned
+  from tensorforge_amd.emit.tuned import Schedule
   w = W.ader_derivative(aligned=False)
   ctx = Context(arch='gfx950', backend='hip', fp_type=FloatingPointType.DOUBLE)
   g = Generator(w.descrs, ctx)
   g.generate()
-  assert g.get_emitted().emitter == 'plain'
-  t = Generator(w.descrs, ctx, emitter='tuned')
-  t.generate()
-  em = t.get_emitted()
+  em = g.get_emitted()
+  assert em.emitter == 'tuned' and em.schedule.get('kind') == 'stream'
+  pl, _ = stream.plan(g.get_program())
+  assert pl.stacked and [t.l_shift for t in pl.terms] == [1, 1, 1]
+  assert [t.L.arg.name for t in pl.terms] == sorted({t.L.arg.name for t in pl.terms},
+                                                    key=[t.L.arg.name for t in pl.terms].index)
+  src = g.get_kernel()
+  # blocks of term 0, then 1, then 2; each term's first block read from its own array
+  assert src.index('of term 0') < src.index('of term 1') < src.index('of term 2')
+  assert 'lp_cur1' in src and 'lp_cur2' in src
+  c = Generator(w.descrs, ctx, emitter='tuned', schedule=Schedule(stream=False))
+  c.generate()
+  em = c.get_emitted()
   assert em.emitter == 'tuned' and em.schedule.get('kind') != 'stream'
   assert tuned.starved(em)
   assert not tuned.starved(_emitted(W.ader_derivative(aligned=True), ctx))
