@@ -59,3 +59,14 @@ def features(generator, candidate, batch, info):
     math.log2(em.lds_bytes + 1), per_block, float(sched.get('loop', False)),
     math.log2(sites), math.log2(groups + 1e-3),
   ]
+
+
+# knobs of the candidate label that the generation-time facts above do not separate
+LABEL_NAMES = ['pair_reads', 'row_loads', 'lds_unpadded', 'k16_blocks']
+
+
+def label_features(label):
+  """Indicator features of a candidate's label (``space.candidate_space`` names)."""
+  parts = label.split('_')
+  return [float('pair' in parts), float('rows' in parts), float('nopad' in parts),
+          float(label.startswith('stream') and 'k16' in parts)]

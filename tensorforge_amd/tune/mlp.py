@@ -67,24 +67,40 @@ class CostMLP:
     self.mean = np.zeros(n_in, dtype=np.float32)
     self.std = np.ones(n_in, dtype=np.float32)
     self.y_mean, self.y_std = 0.0, 1.0
+    self._ws = {}
 
   # ---- raw kernels on device tensors ----
   def _stream(self):
     return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
 
-  def forward_device(self, X):
+  def _buffers(self, B):
+    """Activation, gradient and output buffers for batches of B rows (kept per B: training
+    reuses the same few sizes every epoch)."""
+    bufs = self._ws.get(B)
+    if bufs is None:
+      t = self.torch
+      bufs = (t.empty(3 * B * self.hidden, dtype=t.float32, device=self.device),
+              t.empty(2 * B * self.hidden, dtype=t.float32, device=self.device),
+              t.empty(B, dtype=t.float32, device=self.device))
+      self._ws[B] = bufs
+    return bufs
+
+  def forward_device(self, X, fresh=True):
     B = X.shape[0]
-    work = self.torch.empty(3 * B * self.hidden, dtype=self.torch.float32, device=self.device)
-    out = self.torch.empty(B, dtype=self.torch.float32, device=self.device)
+    if fresh:
+      work = self.torch.empty(3 * B * self.hidden, dtype=self.torch.float32, device=self.device)
+      out = self.torch.empty(B, dtype=self.torch.float32, device=self.device)
+    else:
+      work, _, out = self._buffers(B)
     _check(self.rt.tensorforge_mlp_forward(self.params.data_ptr(), X.data_ptr(), work.data_ptr(),
                                            out.data_ptr(), B, self.n_in, self.hidden,
                                            self._stream()), 'tensorforge_mlp_forward')
     return out, work
 
   def backward_device(self, X, target):
-    out, work = self.forward_device(X)
+    out, work = self.forward_device(X, fresh=False)
     B = X.shape[0]
-    dwork = self.torch.empty(2 * B * self.hidden, dtype=self.torch.float32, device=self.device)
+    dwork = self._buffers(B)[1]
     _check(self.rt.tensorforge_mlp_backward(self.params.data_ptr(), X.data_ptr(), work.data_ptr(),
                                             out.data_ptr(), target.data_ptr(),
                                             self.grads.data_ptr(), dwork.data_ptr(), B, self.n_in,
@@ -116,12 +132,12 @@ class CostMLP:
     n = X.shape[0]
     history = []
     for ep in range(epochs):
+      # one shuffled copy per epoch; the minibatches are contiguous slices of it
       order = self.torch.from_numpy(rng.permutation(n)).to(self.device)
+      Xe = Xd.index_select(0, order)
+      ye = yd.index_select(0, order)
       for s in range(0, n, batch):
-        idx = order[s:s + batch]
-        xb = Xd.index_select(0, idx).contiguous()
-        yb = yd.index_select(0, idx).contiguous()
-        self.backward_device(xb, yb)
+        self.backward_device(Xe[s:s + batch], ye[s:s + batch])
         self.adam_device(lr)
       if ep % max(1, epochs // 10) == 0 or ep == epochs - 1:
         out, _ = self.forward_device(Xd)

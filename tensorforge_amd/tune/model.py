@@ -42,11 +42,18 @@ def split(records, test_fraction=0.25, seed=0):
           [r for r in records if r['chain'] in test])
 
 
+def feature_matrix(records):
+  """Generation-time features of each record plus the indicators of its label's knobs."""
+  from .features import label_features
+  return np.array([list(r['features']) + label_features(r['label']) for r in records],
+                  dtype=np.float64)
+
+
 def arrays(records, centered=True):
   """Features and log(time); ``centered`` subtracts each chain's mean log(time): the choice
   among one chain's candidates only depends on their differences, so the network spends its
   capacity on what the schedule changes rather than on the chain's overall cost."""
-  X = np.array([r['features'] for r in records], dtype=np.float64)
+  X = feature_matrix(records)
   y = np.log(np.array([r['time'] for r in records], dtype=np.float64))
   if centered:
     chains = np.array([r['chain'] for r in records])
@@ -94,7 +101,7 @@ def evaluate(model, records, prune_k=2):
   for chain, recs in sorted(by_chain(records).items()):
     times = np.array([r['time'] for r in recs])
     best = times.min()
-    pred = model.predict(np.array([r['features'] for r in recs]))
+    pred = model.predict(feature_matrix(recs))
     order = np.argsort(pred)
     labels = [r['label'] for r in recs]
     res['model_pick'].append(times[order[0]] / best - 1)
@@ -150,7 +157,7 @@ def cross_validate(records, folds=4, seed=0, prune_k=2, epochs=400, device='cuda
     for chain, rs in sorted(by_chain(recs).items()):
       times = np.array([r['time'] for r in rs])
       best = times.min()
-      order = np.argsort(model.predict(np.array([r['features'] for r in rs])))
+      order = np.argsort(model.predict(feature_matrix(rs)))
       labels = [r['label'] for r in rs]
       out['model_pick'].append(times[order[0]] / best - 1)
       out['pruned_search'].append(times[pruned(order, labels, prune_k)].min() / best - 1)

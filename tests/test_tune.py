@@ -41,7 +41,7 @@ class _Oracle:
   """A 'model' that predicts the measured times exactly: picks must be the sweep's best."""
 
   def __init__(self, recs):
-    self.t = {tuple(r['features']): np.log(r['time']) for r in recs}
+    self.t = {tuple(x): np.log(r['time']) for x, r in zip(M.feature_matrix(recs).tolist(), recs)}
 
   def predict(self, X):
     return np.array([self.t[tuple(x)] for x in X.tolist()])
