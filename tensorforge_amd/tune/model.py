@@ -49,16 +49,26 @@ def feature_matrix(records):
                   dtype=np.float64)
 
 
-def arrays(records, centered=True):
-  """Features and log(time); ``centered`` subtracts each chain's mean log(time): the choice
+def arrays(records, centered=True, target='centered', cap=1.0):
+  """Features and the training target from log(time).
+
+  ``target='centered'`` (or ``centered=True``): log(time) minus the chain's mean: the choice
   among one chain's candidates only depends on their differences, so the network spends its
-  capacity on what the schedule changes rather than on the chain's overall cost."""
+  capacity on what the schedule changes rather than on the chain's overall cost.
+  ``target='gap'``: log(time / the chain's best time), capped at ``cap``: every candidate slower
+  than e^cap times the best looks the same, so the network's accuracy goes to ranking the ones
+  near the best (where a pick is decided). ``target='raw'``: log(time)."""
   X = feature_matrix(records)
   y = np.log(np.array([r['time'] for r in records], dtype=np.float64))
-  if centered:
+  if not centered:
+    target = 'raw'
+  if target in ('centered', 'gap'):
     chains = np.array([r['chain'] for r in records])
     for c in np.unique(chains):
-      y[chains == c] -= y[chains == c].mean()
+      sel = chains == c
+      y[sel] -= y[sel].mean() if target == 'centered' else y[sel].min()
+    if target == 'gap':
+      y = np.minimum(y, cap)
   return X, y
 
 
@@ -73,15 +83,16 @@ class Ensemble:
     return np.mean([m.predict(X) for m in self.models], axis=0)
 
 
-def train(records, epochs=400, lr=2e-3, seed=0, device='cuda:0', centered=True, ensemble=1):
+def train(records, epochs=400, lr=2e-3, seed=0, device='cuda:0', centered=True, ensemble=1,
+          batch=128, target='centered', cap=1.0):
   """A network (``ensemble`` = 1) or an :class:`Ensemble` of ``ensemble`` networks trained on
   ``records``; returns it and the training-loss history of the first network."""
   from .mlp import CostMLP
-  X, y = arrays(records, centered)
+  X, y = arrays(records, centered, target, cap)
   models, history = [], None
   for k in range(ensemble):
     model = CostMLP(X.shape[1], seed=seed + 1000 * k, device=device)
-    h = model.fit(X, y, epochs=epochs, lr=lr, batch=128, seed=seed + 1000 * k)
+    h = model.fit(X, y, epochs=epochs, lr=lr, batch=batch, seed=seed + 1000 * k)
     history = history if history is not None else h
     models.append(model)
   return (models[0] if ensemble == 1 else Ensemble(models)), history
@@ -136,7 +147,7 @@ def pruned(order, labels, k):
 
 
 def cross_validate(records, folds=4, seed=0, prune_k=2, epochs=400, device='cuda:0',
-                   centered=True, ensemble=1):
+                   centered=True, ensemble=1, batch=128, lr=2e-3, target='centered', cap=1.0):
   """Every chain held out once: ``folds`` models (each an ensemble of ``ensemble`` networks),
   each trained on the other chains' timings; the per-chain slowdowns of all folds are pooled
   into one summary."""
@@ -149,8 +160,8 @@ def cross_validate(records, folds=4, seed=0, prune_k=2, epochs=400, device='cuda
   for i, test in enumerate(parts):
     train_recs = [r for r in records if r['chain'] not in test]
     test_recs = [r for r in records if r['chain'] in test]
-    model, _ = train(train_recs, epochs=epochs, seed=seed + i, device=device, centered=centered,
-                     ensemble=ensemble)
+    model, _ = train(train_recs, epochs=epochs, lr=lr, seed=seed + i, device=device,
+                     centered=centered, ensemble=ensemble, batch=batch, target=target, cap=cap)
     pooled.append((model, test_recs))
   out = {'model_pick': [], 'pruned_search': [], 'default': [], 'random_mean': []}
   for model, recs in pooled:
@@ -164,7 +175,8 @@ def cross_validate(records, folds=4, seed=0, prune_k=2, epochs=400, device='cuda
       out['default'].append(times[labels.index(default_choice(labels))] / best - 1)
       out['random_mean'].append(float(np.mean(times / best - 1)))
   summary = {'chains': len(out['model_pick']), 'folds': folds, 'prune_k': prune_k,
-             'ensemble': ensemble, 'epochs': epochs}
+             'ensemble': ensemble, 'epochs': epochs, 'batch': batch, 'lr': lr,
+             'target': target if centered else 'raw'}
   for key, v in out.items():
     v = np.array(v)
     summary[key] = {'mean_slowdown': float(v.mean()), 'median_slowdown': float(np.median(v)),

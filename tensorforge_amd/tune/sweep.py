@@ -131,7 +131,7 @@ def _time_candidate(fn, args, steps, budget_s=0.02):
   e1.record(stream)
   torch.cuda.synchronize()
   first = e0.elapsed_time(e1) / 1e3
-  n = int(min(steps, max(2, budget_s / max(first, 1e-7))))
+  n = int(min(steps, max(3, budget_s / max(first, 1e-7))))
   ev = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
   ev[0].record(stream)
   for j in range(n):
@@ -142,7 +142,7 @@ def _time_candidate(fn, args, steps, budget_s=0.02):
   return ts[len(ts) // 2] if n % 2 else 0.5 * (ts[n // 2 - 1] + ts[n // 2])
 
 
-def measure(directory, steps=7, out_path=None, limit=None):
+def measure(directory, steps=25, out_path=None, limit=None):
   import ctypes
   import torch
   from ..common.context import Context
