@@ -58,26 +58,42 @@ dense_backward_input(const float* __restrict__ W, const float* __restrict__ dZ,
   }
 }
 
-// dW[o][i] = sum_b dZ[b][o] X[b][i]; dbias[o] = sum_b dZ[b][o]. One thread per (o, i) and one per
-// bias entry (index in == i), each summing over the batch in a fixed order.
+// dW[o][i] = sum_b dZ[b][o] X[b][i] (i < in); dbias[o] = sum_b dZ[b][o] (column i == in). One
+// workgroup per 8 x 8 tile of (o, i); its 4 groups of 64 threads each sum every fourth batch row
+// (b = q, q + 4, ...), and the four partial sums are added in a fixed order through LDS, so the
+// result does not depend on scheduling.
 __global__ void __launch_bounds__(kThreads)
 dense_backward_weights(const float* __restrict__ dZ, const float* __restrict__ X,
                        float* __restrict__ dW, float* __restrict__ dbias, int B, int in, int out) {
-  const long total = static_cast<long>(out) * (in + 1);
-  for (long t = blockIdx.x * static_cast<long>(blockDim.x) + threadIdx.x; t < total;
-       t += static_cast<long>(gridDim.x) * blockDim.x) {
-    const int o = static_cast<int>(t / (in + 1));
-    const int i = static_cast<int>(t % (in + 1));
-    float acc = 0.0f;
-    if (i < in) {
-      for (int b = 0; b < B; ++b)
-        acc = fmaf(dZ[static_cast<long>(b) * out + o], X[static_cast<long>(b) * in + i], acc);
-      dW[static_cast<long>(o) * in + i] = acc;
-    } else {
-      for (int b = 0; b < B; ++b) acc += dZ[static_cast<long>(b) * out + o];
-      dbias[o] = acc;
+  __shared__ float part[4][64];
+  const int tiles_i = (in + 1 + 7) / 8;
+  const int to = static_cast<int>(blockIdx.x) / tiles_i;
+  const int ti = static_cast<int>(blockIdx.x) % tiles_i;
+  const int q = static_cast<int>(threadIdx.x) / 64;
+  const int l = static_cast<int>(threadIdx.x) % 64;
+  const int o = to * 8 + l / 8;
+  const int i = ti * 8 + l % 8;
+  const bool valid = o < out && i <= in;
+  float acc = 0.0f;
+  if (valid) {
+    for (int b = q; b < B; b += 4) {
+      const float x = i < in ? X[static_cast<long>(b) * in + i] : 1.0f;
+      acc = fmaf(dZ[static_cast<long>(b) * out + o], x, acc);
     }
   }
+  part[q][l] = acc;
+  __syncthreads();
+  if (q == 0 && valid) {
+    const float sum = ((part[0][l] + part[1][l]) + part[2][l]) + part[3][l];
+    if (i < in)
+      dW[static_cast<long>(o) * in + i] = sum;
+    else
+      dbias[o] = sum;
+  }
+}
+
+unsigned weight_tiles(int in, int out) {
+  return static_cast<unsigned>(((out + 7) / 8) * ((in + 1 + 7) / 8));
 }
 
 // Adam (Kingma & Ba 2015) with bias correction, step t >= 1
@@ -171,23 +187,23 @@ extern "C" int tensorforge_mlp_backward(const float* params, const float* X, con
   dout = dB;
   hipLaunchKernelGGL(loss_grad, dim3(grid_for(B)), dim3(kThreads), 0, s, out, target, dout, B);
   // layer 4: dW4 = dout^T a3, db4; da3 = W4^T dout masked by a3 > 0
-  hipLaunchKernelGGL(dense_backward_weights, dim3(grid_for(hidden + 1)), dim3(kThreads), 0, s,
+  hipLaunchKernelGGL(dense_backward_weights, dim3(weight_tiles(hidden, 1)), dim3(kThreads), 0, s,
                      dout, a3, grads + l.w4, grads + l.b4, B, hidden, 1);
   hipLaunchKernelGGL(dense_backward_input, dim3(grid_for(bh)), dim3(kThreads), 0, s,
                      params + l.w4, dout, a3, dA, B, hidden, 1);
   // layer 3
-  hipLaunchKernelGGL(dense_backward_weights, dim3(grid_for(static_cast<long>(hidden) * (hidden + 1))),
-                     dim3(kThreads), 0, s, dA, a2, grads + l.w3, grads + l.b3, B, hidden, hidden);
+  hipLaunchKernelGGL(dense_backward_weights, dim3(weight_tiles(hidden, hidden)), dim3(kThreads), 0,
+                     s, dA, a2, grads + l.w3, grads + l.b3, B, hidden, hidden);
   hipLaunchKernelGGL(dense_backward_input, dim3(grid_for(bh)), dim3(kThreads), 0, s,
                      params + l.w3, dA, a2, dB, B, hidden, hidden);
   // layer 2
-  hipLaunchKernelGGL(dense_backward_weights, dim3(grid_for(static_cast<long>(hidden) * (hidden + 1))),
-                     dim3(kThreads), 0, s, dB, a1, grads + l.w2, grads + l.b2, B, hidden, hidden);
+  hipLaunchKernelGGL(dense_backward_weights, dim3(weight_tiles(hidden, hidden)), dim3(kThreads), 0,
+                     s, dB, a1, grads + l.w2, grads + l.b2, B, hidden, hidden);
   hipLaunchKernelGGL(dense_backward_input, dim3(grid_for(bh)), dim3(kThreads), 0, s,
                      params + l.w2, dB, a1, dA, B, hidden, hidden);
   // layer 1 (no gradient into the inputs)
-  hipLaunchKernelGGL(dense_backward_weights, dim3(grid_for(static_cast<long>(hidden) * (in + 1))),
-                     dim3(kThreads), 0, s, dA, X, grads + l.w1, grads + l.b1, B, in, hidden);
+  hipLaunchKernelGGL(dense_backward_weights, dim3(weight_tiles(in, hidden)), dim3(kThreads), 0, s,
+                     dA, X, grads + l.w1, grads + l.b1, B, in, hidden);
   return finish("tensorforge_mlp_backward");
 }
 

@@ -70,3 +70,27 @@ def label_features(label):
   parts = label.split('_')
   return [float('pair' in parts), float('rows' in parts), float('nopad' in parts),
           float(label.startswith('stream') and 'k16' in parts)]
+
+
+# occupancy and load balance, derived from the generation-time facts above (a workgroup's LDS
+# bytes, wavefronts and elements, the batch) for an MI355X of 256 CUs with 160 KB of LDS each
+DERIVED_NAMES = ['workgroups_per_cu', 'waves_per_cu', 'log2_rounds', 'last_round_fill']
+
+CUS = 256
+LDS_PER_CU = 160 * 1024
+
+
+def derived_features(v):
+  """Occupancy and load-balance estimates from feature vector ``v`` (``NAMES`` order): resident
+  workgroups per CU by LDS (at most 8), resident wavefronts per CU (at most 32), the number of
+  rounds of workgroups the batch needs on the whole GPU, and how full the last round is."""
+  if len(v) != len(NAMES):       # not a generated feature vector (hand-made records in tests)
+    return [0.0] * len(DERIVED_NAMES)
+  f = dict(zip(NAMES, v))
+  lds = 2.0 ** f['log2_lds_bytes'] - 1.0
+  waves = max(1.0, f['waves'])
+  wg = float(max(1, min(8, int(LDS_PER_CU // lds) if lds >= 1 else 8, int(32 // waves))))
+  batch = 2.0 ** f['log2_batch']
+  rounds = batch / max(1.0, f['elements_per_block']) / (CUS * wg)
+  full = rounds / math.ceil(rounds) if rounds > 0 else 1.0
+  return [wg, wg * waves, math.log2(rounds + 1e-3), full]

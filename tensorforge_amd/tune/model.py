@@ -43,10 +43,11 @@ def split(records, test_fraction=0.25, seed=0):
 
 
 def feature_matrix(records):
-  """Generation-time features of each record plus the indicators of its label's knobs."""
-  from .features import label_features
-  return np.array([list(r['features']) + label_features(r['label']) for r in records],
-                  dtype=np.float64)
+  """Generation-time features of each record, the indicators of its label's knobs and the
+  occupancy and load-balance estimates derived from them."""
+  from .features import derived_features, label_features
+  return np.array([list(r['features']) + label_features(r['label']) +
+                   derived_features(r['features']) for r in records], dtype=np.float64)
 
 
 def arrays(records, centered=True, target='centered', cap=1.0):
