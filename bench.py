@@ -476,7 +476,9 @@ def host_info(threads):
 def cpu_baseline(entry, bufs, gpu_out, budget_s, full=False):
   """The OpenMP restatement of the reference's generated kernel (oracle/omp_chain.cpp, team
   form and flat form) on the first elements of the same inputs, sized to ~budget_s of CPU work;
-  also the GPU result's max normwise error against it on that sample."""
+  also the GPU result's max normwise error against it on that sample, and the same chain's
+  launcher from this package's omptarget backend (``generated``: what ``Context(backend=
+  'omptarget')`` builds from the same descriptions, bit-identical to the flat form)."""
   import numpy as np
   from tensorforge_amd import workloads as W
   lib = ctypes.CDLL(os.path.join(ROOT, 'oracle', 'lib', 'libomp_chain.so'))
@@ -505,10 +507,17 @@ def cpu_baseline(entry, bufs, gpu_out, budget_s, full=False):
   arr_s = (DP * 3)(*[x.ctypes.data_as(DP) for x in S])
   Q = np.ascontiguousarray(host['Q'])
 
+  from tensorforge_amd import catalog
+  _, gen, omp_fn = catalog.load_omp('ader_f64')
+
   def run(form, n):
     out = np.zeros(n * b * q)
     t0 = time.perf_counter()
-    if form == 'team':
+    if form == 'generated':
+      args, _keep = catalog.omp_arguments(gen, dict(host, Qp=out), n)
+      t0 = time.perf_counter()
+      omp_fn(*args)
+    elif form == 'team':
       lib.omp_ader_team(arr_i, arr_c, arr_v, Q.ctypes.data_as(DP), arr_s, out.ctypes.data_as(DP),
                         srow.ctypes.data_as(IP), scol.ctypes.data_as(IP), len(star), b, q,
                         ctypes.c_long(n), 32, 8)
@@ -520,7 +529,8 @@ def cpu_baseline(entry, bufs, gpu_out, budget_s, full=False):
 
   total = len(Q) // (b * q)
   result = {'cores': threads, 'kind': 'port', 'host': host_info(threads)}
-  for form in ('team', 'flat'):
+  flat_out = None
+  for form in ('team', 'flat', 'generated'):
     if full:
       n = total
     else:
@@ -532,7 +542,11 @@ def cpu_baseline(entry, bufs, gpu_out, budget_s, full=False):
       n = int(min(total, max(256, n * budget_s / 2 / max(t, 1e-6))) // 8 * 8)
     t, out = run(form, n)
     result[form] = {'value': n / t, 'elements': n, 'seconds': t}
+    if form == 'generated':
+      result[form]['bitwise_equal_to_flat'] = bool(
+          np.array_equal(out[:min(n, flat_out[1]) * b * q], flat_out[0][:min(n, flat_out[1]) * b * q]))
     if form == 'flat':
+      flat_out = (out, n)
       ref = out.reshape(n, b * q)
       got = gpu_out[:n * b * q].reshape(n, b * q)
       err = np.abs(got - ref).max(axis=1) / np.maximum(np.abs(ref).max(axis=1), 1e-300)
@@ -693,6 +707,7 @@ def main():
                  f"{cpu['team']['seconds']:.1f} s"),
       'host': cpu['host'],
       'flat_form': cpu['flat'],
+      'generated_omptarget': cpu['generated'],
       'gpu_max_normwise_rel_err_vs_cpu': cpu.get('gpu_max_normwise_rel_err')}
   print(json.dumps(line))
 

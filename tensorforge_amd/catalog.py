@@ -98,7 +98,72 @@ def build_catalog(selection=None, jobs=None):
   with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as pool:
     futs = [pool.submit(_build_one, n) for n in todo]
     futs += [pool.submit(build_group, g, part) for g in groups for part in ('tuned', 'plain')]
+    if not selection:
+      futs += [pool.submit(build_omp, (n,)) for n in OMP_ENTRIES]
     return [f.result() for f in futs]
+
+
+# omptarget (host C++/OpenMP) launchers of the CPU-baseline chains, in lib/libomp_<entry>.so
+OMP_ENTRIES = ('ader_f64', 'gemm_f64')
+
+
+def omp_so_path(name):
+  return os.path.join(rt_build.LIB_DIR, f'libomp_{name}.so')
+
+
+def omp_generator(name):
+  """The omptarget backend's generator of catalog chain ``name`` (an ``OMP_ENTRIES`` name: the
+  ``*_tuned`` entry's descriptions), generated."""
+  entry = _REGISTRY[f'{name}_tuned']()
+  gen = Generator(entry.descrs, Context('gfx950', 'omptarget', entry.fp_type))
+  gen.generate()
+  return entry, gen
+
+
+def build_omp(names=OMP_ENTRIES):
+  return [rt_build.compile_host_source(omp_generator(n)[1].get_source(), f'omp_{n}',
+                                       rt_build.LIB_DIR) for n in names]
+
+
+def load_omp(name):
+  """(entry, generator, ctypes function) of a prebuilt omptarget launcher."""
+  import ctypes
+  entry, gen = omp_generator(name)
+  path = omp_so_path(name)
+  if not os.path.exists(path):
+    raise RuntimeError(f'{path} is missing: build the package first')
+  fn = getattr(ctypes.CDLL(path), f'launcher_{gen.get_base_name()}')
+  fn.restype = None
+  return entry, gen, fn
+
+
+def omp_arguments(gen, arrays, n, flags=None, offsets=None):
+  """ctypes arguments of an omptarget launcher over ``n`` elements, in its parameter order:
+  ``arrays`` maps each global tensor's alias to a C-contiguous host array, a list of arrays (a
+  pointer-based tensor: one per element) or a number (a scalar); ``offsets`` the extra offsets
+  (default 0); ``flags`` an optional uint32 array. Returns (arguments, objects to keep alive)."""
+  import ctypes
+  offsets = offsets or {}
+  args, keep = [], []
+  for t in gen._global_tensors:
+    v = arrays[t.alias]
+    if t.addressing == Addressing.SCALAR:
+      args.append(ctypes.c_double(v) if gen._arg_type(t) == F64 else ctypes.c_float(v))
+      continue
+    if isinstance(v, (list, tuple)):
+      table = (ctypes.c_void_p * len(v))(*[x.ctypes.data for x in v])
+      keep.append(table)
+      args.append(ctypes.cast(table, ctypes.c_void_p))
+    else:
+      args.append(ctypes.c_void_p(v.ctypes.data))
+    if t.addressing != Addressing.NONE:
+      args.append(ctypes.c_uint(offsets.get(t.alias, 0)))
+  args.append(ctypes.c_size_t(n))
+  args.append(ctypes.c_void_p(flags.ctypes.data) if flags is not None else ctypes.c_void_p(0))
+  stream = ctypes.c_int(0)                 # streamPtr: a stream object, not read on the host
+  keep.append(stream)
+  args.append(ctypes.cast(ctypes.pointer(stream), ctypes.c_void_p))
+  return args, keep
 
 
 # --------------------------------------------------------------- grouped launches

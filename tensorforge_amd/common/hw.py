@@ -28,14 +28,18 @@ class Gfx950:
 
 
 SUPPORTED_ARCHS = {'gfx950': Gfx950}
-SUPPORTED_BACKENDS = ('hip',)
+# 'hip': the gfx950 kernels; 'omptarget': the same chains as host C++/OpenMP launchers (the CPU
+# form the reference's omptarget backend targets, used for the CPU baseline and parity)
+SUPPORTED_BACKENDS = ('hip', 'omptarget')
 
 
 def hw_descr_factory(arch, backend):
-  """Return the hardware description; anything but gfx950/hip raises ``ValueError`` like the
-  reference's factory does for an unknown architecture (``hw_descr.py:81``)."""
+  """Return the hardware description; anything but gfx950 with the hip or omptarget backend
+  raises ``ValueError`` like the reference's factory does for an unknown architecture
+  (``hw_descr.py:81``)."""
   if backend not in SUPPORTED_BACKENDS:
-    raise ValueError(f'Unknown backend, given: {backend} (only gfx950 HIP is emitted)')
+    raise ValueError(f'Unknown backend, given: {backend} (gfx950 HIP kernels and their '
+                     'omptarget CPU form are emitted)')
   if arch not in SUPPORTED_ARCHS:
     raise ValueError(f'Unknown gpu architecture: {backend} {arch} (only gfx950 is targeted)')
   return SUPPORTED_ARCHS[arch]

@@ -149,6 +149,8 @@ class Generator:
                             'chain they fuse into, or stand alone')
     self._deduce_num_threads()
     from ..emit import plain, tuned, launcher
+    if any(pointwise) and self._context.backend == 'omptarget':
+      raise GenerationError('the omptarget backend generates multilinear chains only')
     if pointwise and all(pointwise):
       from ..emit import pointwise as pw
       if self._emitter_choice not in ('auto', 'tuned', 'plain'):
@@ -163,6 +165,18 @@ class Generator:
       return
     self._program = ir_mod.lower(self.descr_list[:tail], self._global_tensors,
                                  self._context.fp_type)
+    if self._context.backend == 'omptarget':
+      # the CPU form: one host C++/OpenMP launcher with the same name and parameters
+      from ..emit import omp
+      if fused:
+        raise GenerationError('the omptarget backend generates multilinear chains only')
+      self._emitted = launcher.Emitted(kernel_name=f'launcher_{self._base_kernel_name}',
+                                       kernel_src='', block=(1, 1, 1), elements_per_block=1,
+                                       lds_bytes=0, emitter='omptarget')
+      self._kernel = ''
+      self._launcher = omp.emit(self._program, self)
+      self._header = f'{self._generate_launcher_proto(with_defaults=True)};\n'
+      return
     if fused:
       # the statements run on the chain's destination in registers, before it is stored
       from ..emit import pointwise as pw
@@ -228,7 +242,10 @@ class Generator:
     return self._emitted
 
   def get_source(self):
-    """A complete HIP translation unit: includes, kernel, ``extern "C"`` launcher."""
+    """A complete translation unit: includes, kernel, ``extern "C"`` launcher (HIP; host
+    C++/OpenMP for the omptarget backend)."""
+    if self._context.backend == 'omptarget':
+      return self._launcher
     from ..emit import launcher
     return launcher.translation_unit([self])
 

@@ -97,3 +97,24 @@ def compile_generators(generators, name: str, out_dir: str = LIB_DIR, extra_flag
   if not os.path.exists(runtime_path()):
     build_runtime()
   return compile_source(translation_unit(generators), name, out_dir, extra_flags=extra_flags)
+
+
+def host_cxx():
+  path = shutil.which('g++')
+  if path is None:
+    raise RuntimeError('g++ not found; the omptarget launchers need a host C++ compiler')
+  return path
+
+
+def compile_host_source(source: str, name: str, out_dir: str) -> str:
+  """Compile a host C++/OpenMP translation unit (the omptarget backend's launchers) into
+  ``<out_dir>/lib<name>.so``: no fused multiply-add, so each output's sum is rounded term by
+  term as the reference's OpenMP kernels round it."""
+  os.makedirs(out_dir, exist_ok=True)
+  src_path = os.path.join(out_dir, f'{name}.cpp')
+  with open(src_path, 'w') as f:
+    f.write(source)
+  out = os.path.join(out_dir, f'lib{name}.so')
+  _atomic_compile([host_cxx(), '-O2', '-fopenmp', '-ffp-contract=off', '-std=c++17', '-fPIC',
+                   '-shared', src_path], out, f'host build of {name}')
+  return out

@@ -34,7 +34,9 @@ def test_only_gfx950_hip():
   with pytest.raises(ValueError):
     Context('gfx942', 'hip', FloatingPointType.FLOAT)
   with pytest.raises(ValueError):
-    Context('gfx950', 'omptarget', FloatingPointType.FLOAT)
+    Context('gfx950', 'sycl', FloatingPointType.FLOAT)
+  # the host backend: the same descriptions as C++/OpenMP launchers (CPU baseline, hosts)
+  assert Context('gfx950', 'omptarget', FloatingPointType.FLOAT).backend == 'omptarget'
 
 
 def test_align_follows_wavefront_words():
