@@ -21,7 +21,9 @@ DEFAULT_ORDER = ('stream_w8_k8_a2', 'cpl_w1_k16', 'plain')
 
 
 def load(path):
-  with open(path) as f:
+  """Timing records of a database (JSON lines, gzip-compressed when the name ends in .gz)."""
+  import gzip
+  with (gzip.open(path, 'rt') if path.endswith('.gz') else open(path)) as f:
     return [json.loads(line) for line in f if line.strip()]
 
 

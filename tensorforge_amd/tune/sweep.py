@@ -189,6 +189,40 @@ def measure(directory, steps=25, out_path=None, limit=None):
         print(f'{n + 1}/{len(chains)} chains timed ({time.time() - t0:.0f} s)', flush=True)
 
 
+def pack(directories, timing_paths, out_path, label):
+  """One gzip-compressed timing database from sweep directories and their timing files:
+  features rounded to 7 significant digits, times to 6, each record tagged with ``label``
+  (``set``). Returns the sweep's throughput record: chains drawn, kept (at least two candidates)
+  and timed, candidates timed, the build's CPU seconds (generation and compilation, summed over
+  workers) from the directories' indexes."""
+  import gzip
+  recs = []
+  for path in timing_paths:
+    with open(path) as f:
+      recs += [json.loads(line) for line in f if line.strip()]
+  seen = {}
+  for r in recs:                      # a chain timed twice keeps its first timing
+    seen.setdefault((r['chain'], r['label']), r)
+  recs = sorted(seen.values(), key=lambda r: (r['chain'], r['label']))
+  with gzip.open(out_path, 'wt', compresslevel=9) as out:
+    for r in recs:
+      out.write(json.dumps({'chain': r['chain'], 'label': r['label'],
+                            'time': float(f"{r['time']:.6g}"), 'batch': r['batch'],
+                            'info': r['info'],
+                            'features': [float(f'{v:.7g}') for v in r['features']],
+                            'set': label}, separators=(',', ':')) + '\n')
+  meta = {'drawn': 0, 'kept': 0, 'generate_cpu_seconds': 0.0, 'compile_cpu_seconds': 0.0}
+  for d in directories:
+    idx = json.load(open(os.path.join(d, 'index.json')))
+    meta['drawn'] += len(idx['chains']) + idx.get('skipped', 0)
+    meta['kept'] += len(idx['chains'])
+    meta['generate_cpu_seconds'] += idx.get('generate_seconds', 0.0)
+    meta['compile_cpu_seconds'] += idx.get('compile_seconds', 0.0)
+  meta['timed_chains'] = len({r['chain'] for r in recs})
+  meta['timed_candidates'] = len(recs)
+  return meta
+
+
 def main(argv=None):
   ap = argparse.ArgumentParser()
   ap.add_argument('phase', choices=['build', 'measure'])
