@@ -426,14 +426,19 @@ def pointwise_leg(device, steps, warmup, n=262144):
 
 def tuner_leg():
   """Cost model vs the exhaustive sweep: five-fold cross-validation by chain over the
-  committed timing database (218 random chains, every applicable candidate timed on MI355X);
-  the network trains on the GPU in the project's HIP kernels."""
+  committed timing database (every applicable candidate of 2,191 random chains timed twice on
+  MI355X); the networks train on the GPU in the project's HIP kernels. Also the sweep's record:
+  chains drawn and kept, the build's CPU seconds and the measure passes' GPU seconds."""
   from tensorforge_amd.tune import model as M
-  path = os.path.join(ROOT, 'tensorforge_amd', 'tune', 'data', 'timings_mi355x.jsonl')
-  recs = M.load(path)
-  s = M.cross_validate(recs, folds=5, seed=0, prune_k=3, epochs=400, centered=True,
-                       ensemble=5)
+  data = os.path.join(ROOT, 'tensorforge_amd', 'tune', 'data')
+  recs = M.load(os.path.join(data, 'timings_mi355x.jsonl.gz'))
+  t0 = time.time()
+  s = M.cross_validate(recs, folds=5, seed=0, prune_k=3, epochs=150, batch=2048, lr=0.004,
+                       ensemble=8, target='gap', cap=0.25)
+  s['cross_validation_seconds'] = time.time() - t0
   s['candidates_per_chain'] = len(recs) / s['chains']
+  with open(os.path.join(data, 'timings_mi355x.json')) as f:
+    s['sweep'] = json.load(f)
   return s
 
 

@@ -73,8 +73,11 @@ def label_features(label):
 
 
 # occupancy and load balance, derived from the generation-time facts above (a workgroup's LDS
-# bytes, wavefronts and elements, the batch) for an MI355X of 256 CUs with 160 KB of LDS each
-DERIVED_NAMES = ['workgroups_per_cu', 'waves_per_cu', 'log2_rounds', 'last_round_fill']
+# bytes, wavefronts and elements, the batch) for an MI355X of 256 CUs with 160 KB of LDS each;
+# the second set also bounds occupancy by an estimate of the registers a lane holds
+DERIVED_NAMES = ['workgroups_per_cu', 'waves_per_cu', 'log2_rounds', 'last_round_fill',
+                 'log2_vgpr_estimate', 'waves_per_simd_by_vgpr', 'workgroups_per_cu_by_vgpr',
+                 'log2_rounds_by_vgpr', 'last_round_fill_by_vgpr']
 
 CUS = 256
 LDS_PER_CU = 160 * 1024
@@ -83,7 +86,9 @@ LDS_PER_CU = 160 * 1024
 def derived_features(v):
   """Occupancy and load-balance estimates from feature vector ``v`` (``NAMES`` order): resident
   workgroups per CU by LDS (at most 8), resident wavefronts per CU (at most 32), the number of
-  rounds of workgroups the batch needs on the whole GPU, and how full the last round is."""
+  rounds of workgroups the batch needs on the whole GPU, and how full the last round is; then
+  the same with occupancy also bounded by registers (the estimate, wavefronts per SIMD,
+  workgroups per CU, whole rounds of the persistent grid, fill of the last round)."""
   if len(v) != len(NAMES):       # not a generated feature vector (hand-made records in tests)
     return [0.0] * len(DERIVED_NAMES)
   f = dict(zip(NAMES, v))
@@ -91,6 +96,19 @@ def derived_features(v):
   waves = max(1.0, f['waves'])
   wg = float(max(1, min(8, int(LDS_PER_CU // lds) if lds >= 1 else 8, int(32 // waves))))
   batch = 2.0 ** f['log2_batch']
-  rounds = batch / max(1.0, f['elements_per_block']) / (CUS * wg)
+  groups = batch / max(1.0, f['elements_per_block'])
+  rounds = groups / (CUS * wg)
   full = rounds / math.ceil(rounds) if rounds > 0 else 1.0
-  return [wg, wg * waves, math.log2(rounds + 1e-3), full]
+  # registers: the column-per-lane and streaming forms keep the lane's destination column
+  # (rows x 2 words in FP64) plus a fixed share for streamed rows, addresses and temporaries;
+  # 512 per SIMD lane give the resident wavefronts per SIMD (at most 8), four SIMDs per CU
+  words = 2 if f['fp64'] else 1
+  vg = 64.0 if f['em_plain'] else f['max_rows'] * words + (40 if f['em_stream'] else 24)
+  wps = max(1, min(8, int(512 // max(vg, 24.0))))
+  wg_v = max(1, min(int(LDS_PER_CU // lds) if lds >= 1 else 32,
+                    int(wps * 4 // waves) if waves <= wps * 4 else 1))
+  grid = CUS * wg_v
+  rounds_v = math.ceil(groups / grid) if groups > 0 else 1
+  fill_v = groups / (rounds_v * grid)
+  return [wg, wg * waves, math.log2(rounds + 1e-3), full,
+          math.log2(vg), float(wps), float(wg_v), math.log2(rounds_v), fill_v]

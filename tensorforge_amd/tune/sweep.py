@@ -192,25 +192,34 @@ def measure(directory, steps=25, out_path=None, limit=None):
 def pack(directories, timing_paths, out_path, label):
   """One gzip-compressed timing database from sweep directories and their timing files:
   features rounded to 7 significant digits, times to 6, each record tagged with ``label``
-  (``set``). Returns the sweep's throughput record: chains drawn, kept (at least two candidates)
+  (``set``). A candidate timed in several files (separate measure passes) keeps the mean of its
+  times and their count (``runs``): the sweep's best of a chain is then less often a lucky
+  launch. Returns the sweep's throughput record: chains drawn, kept (at least two candidates)
   and timed, candidates timed, the build's CPU seconds (generation and compilation, summed over
   workers) from the directories' indexes."""
   import gzip
-  recs = []
+  times, first = {}, {}
   for path in timing_paths:
     with open(path) as f:
-      recs += [json.loads(line) for line in f if line.strip()]
-  seen = {}
-  for r in recs:                      # a chain timed twice keeps its first timing
-    seen.setdefault((r['chain'], r['label']), r)
-  recs = sorted(seen.values(), key=lambda r: (r['chain'], r['label']))
+      for line in f:
+        if line.strip():
+          r = json.loads(line)
+          key = (r['chain'], r['label'])
+          times.setdefault(key, []).append(r['time'])
+          first.setdefault(key, r)
+  recs = []
+  for key in sorted(first):
+    r = dict(first[key])
+    r['time'] = sum(times[key]) / len(times[key])
+    r['runs'] = len(times[key])
+    recs.append(r)
   with gzip.open(out_path, 'wt', compresslevel=9) as out:
     for r in recs:
       out.write(json.dumps({'chain': r['chain'], 'label': r['label'],
                             'time': float(f"{r['time']:.6g}"), 'batch': r['batch'],
                             'info': r['info'],
                             'features': [float(f'{v:.7g}') for v in r['features']],
-                            'set': label}, separators=(',', ':')) + '\n')
+                            'runs': r['runs'], 'set': label}, separators=(',', ':')) + '\n')
   meta = {'drawn': 0, 'kept': 0, 'generate_cpu_seconds': 0.0, 'compile_cpu_seconds': 0.0}
   for d in directories:
     idx = json.load(open(os.path.join(d, 'index.json')))
@@ -225,8 +234,8 @@ def pack(directories, timing_paths, out_path, label):
 
 def main(argv=None):
   ap = argparse.ArgumentParser()
-  ap.add_argument('phase', choices=['build', 'measure'])
-  ap.add_argument('--dir', required=True)
+  ap.add_argument('phase', choices=['build', 'measure', 'pack'])
+  ap.add_argument('--dir', required=True, help='sweep directory (pack: comma-separated list)')
   ap.add_argument('--chains', type=int, default=150)
   ap.add_argument('--start', type=int, default=0, help='index of the first chain drawn')
   ap.add_argument('--seed', type=int, default=5)
@@ -234,12 +243,30 @@ def main(argv=None):
   ap.add_argument('--max-source', type=int, default=MAX_SOURCE)
   ap.add_argument('--precision', choices=['exact', 'mixed'], default='exact')
   ap.add_argument('--limit', type=int, default=None, help='time at most this many chains')
-  ap.add_argument('--out', default=None, help='timings file (default: DIR/timings.jsonl)')
+  ap.add_argument('--out', default=None, help='timings file (default: DIR/timings.jsonl; pack: '
+                  'the database, .jsonl.gz)')
+  ap.add_argument('--timings', default='', help='pack: comma-separated timing files')
+  ap.add_argument('--logs', default='', help='pack: comma-separated measure logs (their last '
+                  'line gives the pass\'s chains and GPU seconds)')
+  ap.add_argument('--label', default='sweep', help='pack: the records\' set name')
   a = ap.parse_args(argv)
   if a.phase == 'build':
     build(a.dir, a.chains, a.seed, a.jobs, a.max_source, a.precision, a.start)
-  else:
+  elif a.phase == 'measure':
     measure(a.dir, out_path=a.out, limit=a.limit)
+  else:
+    meta = pack(a.dir.split(','), a.timings.split(','), a.out, a.label)
+    passes = []
+    for path in filter(None, a.logs.split(',')):
+      with open(path) as f:
+        last = [line for line in f if 'chains timed' in line][-1]
+      done, secs = last.split('/')[0], last.split('(')[1].split()[0]
+      passes.append({'chains': int(done), 'seconds': float(secs)})
+    meta['measure_passes'] = passes
+    meta['seed'] = a.seed
+    with open(a.out.replace('.jsonl.gz', '.json'), 'w') as f:
+      json.dump(meta, f, indent=1)
+    print(json.dumps(meta))
 
 
 if __name__ == '__main__':

@@ -109,3 +109,21 @@ def test_sweep_chain_draws_are_reproducible_per_index():
   c = sweep.chain_descriptions(5, 124)
   assert a[3] == b[3] and a[2] == b[2]
   assert (a[3], a[2]) != (c[3], c[2])
+
+
+def test_timing_database_size_and_sweep_record():
+  """The committed timing database: at least 2,000 timed chains, every candidate timed in two
+  passes, counts matching the sweep's record."""
+  import json
+  import os
+  from tensorforge_amd.tune import model as M
+  data = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                      'tensorforge_amd', 'tune', 'data')
+  recs = M.load(os.path.join(data, 'timings_mi355x.jsonl.gz'))
+  meta = json.load(open(os.path.join(data, 'timings_mi355x.json')))
+  chains = {r['chain'] for r in recs}
+  assert len(chains) == meta['timed_chains'] >= 2000
+  assert len(recs) == meta['timed_candidates']
+  assert all(r['runs'] == 2 for r in recs)
+  assert meta['drawn'] >= meta['kept'] >= meta['timed_chains']
+  assert sum(p['chains'] for p in meta['measure_passes']) == 2 * meta['timed_chains']
