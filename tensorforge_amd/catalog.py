@@ -139,17 +139,29 @@ def load_omp(name):
 
 def omp_arguments(gen, arrays, n, flags=None, offsets=None):
   """ctypes arguments of an omptarget launcher over ``n`` elements, in its parameter order:
-  ``arrays`` maps each global tensor's alias to a C-contiguous host array, a list of arrays (a
-  pointer-based tensor: one per element) or a number (a scalar); ``offsets`` the extra offsets
-  (default 0); ``flags`` an optional uint32 array. Returns (arguments, objects to keep alive)."""
+  ``arrays`` maps each global tensor's alias (its generated name when it has none) to a
+  C-contiguous host array, a list of arrays (a pointer-based tensor: one per element) or a number
+  (a floating-point scalar); ``offsets`` the extra offsets by the same keys (default 0); ``flags``
+  an optional uint32 array. Returns (arguments, objects to keep alive)."""
   import ctypes
   offsets = offsets or {}
   args, keep = [], []
   for t in gen._global_tensors:
-    v = arrays[t.alias]
+    key = t.alias if t.alias is not None else t.name
     if t.addressing == Addressing.SCALAR:
-      args.append(ctypes.c_double(v) if gen._arg_type(t) == F64 else ctypes.c_float(v))
+      ty = gen._arg_type(t)
+      if ty not in (F64, F32):
+        raise TypeError(f'scalar {key}: only floating-point scalars are passed here')
+      v = arrays[key] if t.data is None else \
+          (float(t.data[0]) if not isinstance(t.data, dict) else 0.0)
+      args.append(ctypes.c_double(v) if ty == F64 else ctypes.c_float(v))
       continue
+    if t.has_values() and len(t.get_values()) < 16 and key not in arrays:
+      args.append(ctypes.c_void_p(0))        # values baked into the launcher: not read
+      if t.addressing != Addressing.NONE:
+        args.append(ctypes.c_uint(0))
+      continue
+    v = arrays[key]
     if isinstance(v, (list, tuple)):
       table = (ctypes.c_void_p * len(v))(*[x.ctypes.data for x in v])
       keep.append(table)
@@ -157,7 +169,7 @@ def omp_arguments(gen, arrays, n, flags=None, offsets=None):
     else:
       args.append(ctypes.c_void_p(v.ctypes.data))
     if t.addressing != Addressing.NONE:
-      args.append(ctypes.c_uint(offsets.get(t.alias, 0)))
+      args.append(ctypes.c_uint(offsets.get(key, 0)))
   args.append(ctypes.c_size_t(n))
   args.append(ctypes.c_void_p(flags.ctypes.data) if flags is not None else ctypes.c_void_p(0))
   stream = ctypes.c_int(0)                 # streamPtr: a stream object, not read on the host
