@@ -320,8 +320,8 @@ def deriv_leg(device, steps, warmup, n=262144):
 
 def gemm_leg(device, steps, warmup, cpu_seconds, with_cpu, n=65536):
   """Single batched GEMM C_e += A * B_e (A 56x56 shared, B_e and C_e 56x9, seed 0, FP64,
-  65,536 elements): tuned and plain launchers with the HBM roofline (12,096 B per element), the
-  tuned result against the closed form on every element, and the OpenMP restatement of the
+  65,536 elements): tuned (streaming), plain and FP64 matrix-core launchers with the HBM
+  roofline (12,096 B per element), each result against the closed form on every element, and the OpenMP restatement of the
   reference's generated kernel (oracle/omp_chain.cpp, team and flat forms) on the host."""
   import numpy as np
   import torch
@@ -330,6 +330,7 @@ def gemm_leg(device, steps, warmup, cpu_seconds, with_cpu, n=65536):
   from tensorforge_amd.common.basic_types import FloatingPointType
   entry, tuned = catalog.load('gemm_f64_tuned')
   _, plain = catalog.load('gemm_f64_plain')
+  _, mfma64 = catalog.load('gemm_f64_mfma_fp64')
   w = W.gemm(FloatingPointType.DOUBLE)
   host = W.gemm_inputs(w, n, seed=0)
   b, q = w.meta['b'], w.meta['q']
@@ -338,7 +339,7 @@ def gemm_leg(device, steps, warmup, cpu_seconds, with_cpu, n=65536):
   res = {'elements': n, 'unit': 'elements/s'}
   em = tuned.generator.get_emitted()
   bpl = em.bytes_per_element * n + em.shared_bytes
-  for label, k in (('tuned', tuned), ('plain', plain)):
+  for label, k in (('tuned', tuned), ('plain', plain), ('mfma_fp64', mfma64)):
     bufs = {key: torch.from_numpy(v.copy()).to(device) for key, v in host.items()}
     from tensorforge_amd.device import launch
     launch(k, bufs, n)
@@ -352,7 +353,9 @@ def gemm_leg(device, steps, warmup, cpu_seconds, with_cpu, n=65536):
                   'max_normwise_rel_err_vs_closed_form': err}
     del bufs
   res['tuned_over_plain'] = res['plain']['ms_per_step'] / res['tuned']['ms_per_step']
+  res['mfma_fp64_over_plain'] = res['plain']['ms_per_step'] / res['mfma_fp64']['ms_per_step']
   res['schedule'] = em.schedule
+  res['mfma_fp64_schedule'] = mfma64.generator.get_emitted().schedule
   if with_cpu:
     lib = ctypes.CDLL(os.path.join(ROOT, 'oracle', 'lib', 'libomp_chain.so'))
     DP = ctypes.POINTER(ctypes.c_double)
@@ -608,6 +611,21 @@ def main():
       bufs_pd[f'K{d + 1}'] = dense
     bufs_pd['Qp'] = torch.zeros_like(bufs['Qp'])
     t_pd = time_kernel(plain_dense, bufs_pd, n, args.steps, args.warmup)
+    pd_out = bufs_pd['Qp'].cpu().numpy().reshape(n, -1)
+    # the same dense-K chain in the streaming form (one DPP FMA per entry, zeros included) and in
+    # the FP64 matrix-core form, each against the plain dense-K result
+    dense_forms = {}
+    for label, name in (('tuned', 'ader_dense_f64_tuned'), ('mfma_fp64', 'ader_dense_f64_mfma_fp64')):
+      _, kd = catalog.load(name)
+      bufs_pd['Qp'] = torch.zeros_like(bufs['Qp'])
+      t_d = time_kernel(kd, bufs_pd, n, args.steps, args.warmup)
+      got_d = bufs_pd['Qp'].cpu().numpy().reshape(n, -1)
+      diff = float((np.abs(got_d - pd_out).max(axis=1) /
+                    np.maximum(np.abs(pd_out).max(axis=1), 1e-300)).max())
+      dense_forms[label] = {'value': n / t_d, 'ms_per_step': t_d * 1e3,
+                            'over_plain_dense_k': t_pd / t_d,
+                            'max_normwise_rel_diff_vs_plain_dense_k': diff,
+                            'schedule': kd.generator.get_emitted().schedule}
     del bufs_pd
 
   # the headline kernel's result (one untimed launch), compared with the timed launches' result
@@ -659,7 +677,8 @@ def main():
                     'tuned_over_plain': t_plain / t_tuned,
                     'max_normwise_rel_diff_tuned_vs_plain': agree}
     out['plain_dense_k'] = {'value': n / t_pd, 'ms_per_step': t_pd * 1e3,
-                            'tuned_over_plain_dense_k': t_pd / t_tuned}
+                            'tuned_over_plain_dense_k': t_pd / t_tuned,
+                            'dense_k_forms': dense_forms}
 
   if 'pointwise' in legs:
     out['pointwise'] = pointwise_leg(device, args.steps, args.warmup)

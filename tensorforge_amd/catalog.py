@@ -287,6 +287,12 @@ for _emitter in ('plain', 'tuned'):
     lambda e=_emitter: _wl_entry(W.ader_derivative(F64, aligned=False), e))
 
 
+# FP64 matrix-core form of the dense chains: the single GEMM, the order-6 volume chain with
+# dense K_d (its shared operands fit LDS as doubles in 8-row blocks)
+register('gemm_f64_mfma_fp64')(lambda: _wl_entry(W.gemm(F64), 'mfma_fp64'))
+register('ader_dense_f64_mfma_fp64')(
+  lambda: _wl_entry(W.ader_volume(F64, sparse_k=False), 'mfma_fp64'))
+
 # precision study: order 7 (84 basis functions), dense K_d, FP64 inputs and outputs
 for _emitter in ('plain', 'tuned', 'mfma_fp32', 'mfma_bf16x3'):
   register(f'ader7_f64_{_emitter}')(

@@ -1,13 +1,17 @@
-"""Matrix-core forms of the streaming chains (precision study): FP32 MFMA with FP32 accumulation,
-and split BF16x3 MFMA with a compensating second accumulator.
+"""Matrix-core forms of the streaming chains: FP64 MFMA (``v_mfma_f64_16x16x4f64``, the same
+arithmetic type as the vector form, for shared operands dense enough that their padded 16x16
+tiles cost less than one DPP FMA per stored entry), and for the precision study FP32 MFMA with
+FP32 accumulation and split BF16x3 MFMA with a compensating second accumulator.
 
 Applies to the chains the streaming schedule recognises, with dense shared operands::
 
     D_e = sum_t U_t · (L_e · W_t,e)          (U_t shared b x b, L_e per element b x q,
                                                W_t,e per element q x q, dense or sparse)
+    D_e = [D_e +] U · B_e                    (one product with a per-element right operand)
 
 The per-element temporaries ``L_e · W_t,e`` (the star-matrix step) stay in FP64 on the vector
-units; the products with the shared matrices run on MFMA. Inputs and outputs keep the FP64 layout
+units; the products with the shared matrices run on MFMA. In FP64 the shared operands sit in LDS
+as doubles, so the contraction blocks shrink to 8 rows when 16 would not fit. Inputs and outputs keep the FP64 layout
 and launcher of the chain, so the launcher is a drop-in for the FP64 one.
 
 Mapping: a workgroup of 4 wavefronts stays resident; each wavefront owns ``64 // q`` elements, i.e.
@@ -37,21 +41,59 @@ WAVES = 4
 KB = 16            # contraction rows per block
 
 
-def unsupported_reason(program, pl):
+LDS_LIMIT = 160 * 1024
+
+
+def _shape(pl, mode):
+  """(wavefronts per workgroup, contraction rows per block). The FP32 and BF16x3 forms: 4 and 16.
+  FP64 with a per-element right operand (the GEMM): eight wavefronts (two per SIMD, so one's
+  memory waits overlap the other's MFMAs; 224 registers) with blocks of 8 rows, whose B tiles are
+  half as large. FP64 chains that form temporaries hold their weights and bpermute sources as
+  well and keep four wavefronts (two would spill), with 16-row blocks when they fit in LDS."""
+  if mode != 'fp64':
+    return WAVES, KB
+  shapes = ((8, 8), (4, 16), (4, 8)) if pl.terms[0].kind == 'global' else ((4, 16), (4, 8))
+  for waves, kb in shapes:
+    if _lds_bytes(pl, mode, kb, waves) <= LDS_LIMIT:
+      return waves, kb
+  return WAVES, 8
+
+
+def _lds_bytes(pl, mode, kb, waves=WAVES):
+  """LDS of the form: the shared operands in A-operand order plus two B tiles per wavefront."""
+  K = pl.khi - pl.klo
+  ks = (K + kb - 1) // kb * (kb // 4 if mode != 'bf16x3' else kb // 16)
+  rt = (pl.rows[1] - pl.rows[0] + 15) // 16
+  frag = len(pl.terms) * ks * rt * WAVE
+  if mode == 'bf16x3':
+    return frag * 16 + waves * 2 * 64 * 20 * 4
+  if mode == 'fp64':
+    return frag * 8 + waves * 2 * 64 * (kb + 1) * 8
+  return frag * 4 + waves * 2 * 64 * 17 * 4
+
+
+def unsupported_reason(program, pl, mode='fp32'):
   if pl is None:
     return 'not a streaming chain'
   if program.fp_type != FloatingPointType.DOUBLE:
     return 'the matrix-core forms take FP64 chains'
-  if any(t.kind != 'fused' for t in pl.terms):
+  kinds = {t.kind for t in pl.terms}
+  if kinds == {'global'}:
+    if len(pl.terms) != 1:
+      return 'more than one product with a per-element right operand'
+  elif kinds != {'fused'}:
     return 'a term without a per-element temporary'
   if any(t.U.is_sparse() for t in pl.terms):
     return 'sparse shared operand'
   if any(tuple(t.product.k_ranges[0]) != (pl.klo, pl.khi) for t in pl.terms):
     return 'terms with different contraction ranges'
-  if pl.L is None or pl.L.arg.addressing != Addressing.STRIDED:
-    return 'streamed operand is not strided'
-  if pl.l_cols[1] - pl.l_cols[0] != pl.q or pl.q > 16:
-    return 'temporary columns'
+  if kinds == {'fused'}:
+    if pl.L is None or pl.L.arg.addressing != Addressing.STRIDED:
+      return 'streamed operand is not strided'
+    if pl.l_cols[1] - pl.l_cols[0] != pl.q:
+      return 'temporary columns'
+  if pl.q > 16:
+    return 'more than 16 columns'
   if pl.rows[1] - pl.rows[0] > 128 or pl.khi - pl.klo > 128:
     return 'more than 128 rows'
   if pl.init == 'load' and pl.beta != 1.0:
@@ -68,6 +110,8 @@ def unsupported_reason(program, pl):
     return 'a per-element operand is not strided'
   if program.epilogue is not None:
     return 'pointwise statements fused into the chain'
+  if _lds_bytes(pl, mode, _shape(pl, mode)[1], _shape(pl, mode)[0]) > LDS_LIMIT:
+    return 'shared operands do not fit in LDS'
   return None
 
 
@@ -84,11 +128,18 @@ class _Mfma:
     self.rows = pl.rows[1] - pl.rows[0]
     self.RT = (self.rows + 15) // 16
     self.K = pl.khi - pl.klo
-    self.NKB = (self.K + KB - 1) // KB
-    self.kstep = 4 if mode == 'fp32' else 16      # k per MFMA: 16x16x4 f32, 16x16x16 bf16
-    self.KS = self.NKB * (KB // self.kstep)
+    self.waves, self.KB = _shape(pl, mode)
+    self.NKB = (self.K + self.KB - 1) // self.KB
+    # k per MFMA: 16x16x4 f64 and f32, 16x16x16 bf16
+    self.kstep = 16 if mode == 'bf16x3' else 4
+    self.KS = self.NKB * (self.KB // self.kstep)
     self.T = len(pl.terms)
     self.prefetch_rows = mode != 'bf16x3'
+    # one product with a per-element right operand: its rows go to the B tile as they are
+    self.B = pl.terms[0].B if pl.terms[0].kind == 'global' else None
+    self.real = 'double' if mode == 'fp64' else 'float'
+    # FP64 accumulates into the destination's old values (loaded at the start of the group)
+    self.preload = mode == 'fp64' and pl.init == 'load'
 
   # --------------------------------------------------------------- per-lane star slots
   def slot_tables(self, t):
@@ -130,20 +181,29 @@ class _Mfma:
     pl = self.pl
     T, KS, RT, CT = self.T, self.KS, self.RT, self.CT
     bf16 = self.mode == 'bf16x3'
-    threads = WAVE * WAVES
+    fp64 = self.mode == 'fp64'
+    threads = WAVE * self.waves
     frag = T * KS * RT * WAVE
     if bf16:
       # FP32 in LDS, split into BF16 terms when read: per lane and fragment the 4 consecutive
       # contraction entries of its row (A) or column (B), one 16-byte read
       kbytes = frag * 16
       pitch = 20                          # floats per column of the B tile (16-byte rows)
+      tbytes = 64 * pitch * 4
+    elif fp64:
+      kbytes = frag * 8
+      pitch = self.KB + 1                 # doubles per column of the B tile
+      tbytes = 64 * pitch * 8
     else:
       kbytes = frag * 4
       pitch = 17
-    tbytes = 64 * pitch * 4
-    lds_bytes = kbytes + WAVES * 2 * tbytes
+      tbytes = 64 * pitch * 4
+    lds_bytes = kbytes + self.waves * 2 * tbytes
+    assert lds_bytes == _lds_bytes(pl, self.mode, self.KB, self.waves)
     self.lds_bytes = lds_bytes
     w('typedef float tf_f4 __attribute__((ext_vector_type(4)));')
+    if fp64:
+      w('typedef double tf_d4 __attribute__((ext_vector_type(4)));')
     w('typedef short tf_s4 __attribute__((ext_vector_type(4)));')
     w('#ifndef TF_BF16_SPLIT3')
     w('#define TF_BF16_SPLIT3')
@@ -176,18 +236,18 @@ class _Mfma:
     w(f'extern "C" __global__ void __launch_bounds__({threads}, 1) {self.kname}({params})')
     w('{')
     w._depth += 1
-    w(f'// matrix-core form ({self.mode}): {WAVES} wavefronts, {self.G} elements x {self.q} '
-      f'columns each, {RT}x{CT} tiles of 16x16, contraction blocks of {KB}')
+    w(f'// matrix-core form ({self.mode}): {self.waves} wavefronts, {self.G} elements x {self.q} '
+      f'columns each, {RT}x{CT} tiles of 16x16, contraction blocks of {self.KB}')
     w('extern __shared__ __align__(16) unsigned char lds_raw[];')
     if bf16:
       w('float4* __restrict__ kf4 = reinterpret_cast<float4*>(lds_raw);')
     else:
-      w('float* __restrict__ kf = reinterpret_cast<float*>(lds_raw);')
+      w(f'{self.real}* __restrict__ kf = reinterpret_cast<{self.real}*>(lds_raw);')
     w('const int wave = threadIdx.x / 64;')
     w('const int lane = threadIdx.x % 64;')
     for b in range(2):
       base = f'lds_raw + {kbytes} + (wave * 2 + {b}) * {tbytes}'
-      w(f'float* __restrict__ tf{b} = reinterpret_cast<float*>({base});')
+      w(f'{self.real}* __restrict__ tf{b} = reinterpret_cast<{self.real}*>({base});')
     self.stage_shared(frag, threads)
     w('__syncthreads();')
     q, G = self.q, self.G
@@ -198,6 +258,9 @@ class _Mfma:
     self.tables = []
     src_names = {}
     for ti, t in enumerate(pl.terms):
+      if t.kind != 'fused':
+        self.tables.append([])
+        continue
       tabs = []
       for si, (m, p) in enumerate(self.slot_tables(t)):
         key = tuple(m)
@@ -210,17 +273,17 @@ class _Mfma:
         tabs.append((src_names[key], pv))
       self.tables.append(tabs)
     w(f'const size_t ngroups = (numElements + {G - 1}) / {G};')
-    w(f'const size_t stride = (size_t)gridDim.x * {WAVES};')
-    with w.block('for (size_t grp = (size_t)blockIdx.x * ' + str(WAVES) + ' + wave; grp < ngroups; '
+    w(f'const size_t stride = (size_t)gridDim.x * {self.waves};')
+    with w.block('for (size_t grp = (size_t)blockIdx.x * ' + str(self.waves) + ' + wave; grp < ngroups; '
                  'grp += stride)'):
       self.group(pitch)
     w._depth -= 1
     w('}')
     traffic = element_traffic(self.prog)
-    sched = {'kind': f'mfma_{self.mode}', 'waves_per_block': WAVES, 'elements_per_wave': G,
-             'row_tiles': RT, 'column_tiles': CT, 'k_block': KB, 'lds_bytes': lds_bytes}
+    sched = {'kind': f'mfma_{self.mode}', 'waves_per_block': self.waves, 'elements_per_wave': G,
+             'row_tiles': RT, 'column_tiles': CT, 'k_block': self.KB, 'lds_bytes': lds_bytes}
     return Emitted(kernel_name=self.kname, kernel_src=w.src(), block=(threads, 1, 1),
-                   elements_per_block=G * WAVES, lds_bytes=lds_bytes, emitter='mfma',
+                   elements_per_block=G * self.waves, lds_bytes=lds_bytes, emitter='mfma',
                    schedule=sched, bytes_per_element=traffic.per_element,
                    shared_bytes=traffic.shared, grid_override='tf_grid',
                    extra_launch_setup=self.grid_setup(lds_bytes, threads))
@@ -247,7 +310,9 @@ class _Mfma:
       w(f'const int ks = (e / {64 * RT}) % {KS};')
       w(f'const int t = e / {64 * RT * KS};')
       w('const int row = rt * 16 + l % 16;')
-      if self.mode == 'fp32':
+      if self.mode == 'fp64':
+        w(f'kf[e] = {self.shared_value("t", "row", "ks * 4 + l / 16")};')
+      elif self.mode == 'fp32':
         w(f'kf[e] = (float){self.shared_value("t", "row", "ks * 4 + l / 16")};')
       else:
         w('float x[4];')
@@ -272,30 +337,42 @@ class _Mfma:
       ty = f'{"const " if a.const else ""}{a.dtype}'
       w(f'{ty}* __restrict__ g_{a.name} = {a.name} + eSafe * {a.element_stride}ull + '
         f'{a.name}_extraOffset;')
-    w(f'const double* __restrict__ lq = g_{L.arg.name} + (col + {pl.l_cols[0] - L.origin[1]}) * '
-      f'{L.strides[1]};')
+    if self.B is not None:
+      # the lane's column of the per-element right operand
+      B = self.B
+      kd, nd = pl.terms[0].b_dims
+      w(f'const double* __restrict__ lq = g_{B.arg.name} + (col + {pl.cols[0] - B.origin[nd]}) * '
+        f'{B.strides[nd]};')
+    else:
+      w(f'const double* __restrict__ lq = g_{L.arg.name} + (col + {pl.l_cols[0] - L.origin[1]}) '
+        f'* {L.strides[1]};')
     # weights of every slot of every term (FP64)
     for ti, t in enumerate(pl.terms):
       for si, (src, pv) in enumerate(self.tables[ti]):
         w(f'const double w{ti}_{si} = {pv} >= 0 ? (double)g_{t.W.arg.name}[{pv} < 0 ? 0 : {pv}] '
           f': 0.0;')
-    w('tf_f4 acc[' + str(RT * CT) + '];')
+    acc_t = 'tf_d4' if self.mode == 'fp64' else 'tf_f4'
+    w(f'{acc_t} acc[{RT * CT}];')
     if bf16:
       w('tf_f4 acr[' + str(RT * CT) + '];')
     w('#pragma unroll')
     with w.block(f'for (int i = 0; i < {RT * CT}; ++i)'):
-      w('acc[i] = tf_f4{0.f, 0.f, 0.f, 0.f};')
+      w(f'acc[i] = {acc_t}{{0.0, 0.0, 0.0, 0.0}};' if self.mode == 'fp64' else
+        'acc[i] = tf_f4{0.f, 0.f, 0.f, 0.f};')
       if bf16:
         w('acr[i] = tf_f4{0.f, 0.f, 0.f, 0.f};')
+    if self.preload:
+      self.load_destination()
     # software pipeline over the (block, term) steps: the MFMAs of step s read B tile s % 2
     # while the lanes form step s + 1's temporary rows into the other tile; the streamed rows of
     # block kb + 1 are loaded when block kb starts
     steps = [(kb, t) for kb in range(self.NKB) for t in range(self.T)]
     # without prefetch the next block's rows are loaded one step ahead (fewer live registers)
     nxt_loads = lambda si: (si + 1 < len(steps) and steps[si + 1][0] != steps[si][0])
-    w(f'double qa[{KB}], qb[{KB}];')
+    KBr = self.KB
+    w(f'double qa[{KBr}], qb[{KBr}];')
     self.load_rows(0, 'qa')
-    self.t_rows(steps[0], 'qa', 0, pitch, range(KB))
+    self.t_rows(steps[0], 'qa', 0, pitch, range(KBr))
     self.tile_fence()
     for si, (kb, t) in enumerate(steps):
       qcur = 'qa' if kb % 2 == 0 else 'qb'
@@ -307,8 +384,8 @@ class _Mfma:
         self.load_rows(kb + 1, qnxt)
       nxt = steps[si + 1] if si + 1 < len(steps) else None
       mf = self.mfma_list(kb, t, si % 2, pitch)
-      per = (len(mf) + KB - 1) // KB
-      for j in range(KB):
+      per = (len(mf) + KBr - 1) // KBr
+      for j in range(KBr):
         if nxt is not None:
           self.t_rows(nxt, qcur if nxt[0] == kb else qnxt, (si + 1) % 2, pitch, [j])
         for line in mf[j * per:(j + 1) * per]:
@@ -321,11 +398,15 @@ class _Mfma:
   def load_rows(self, kb, dst):
     w = self.w
     pl = self.pl
-    L = pl.L
-    for j in range(KB):
-      k = kb * KB + j
+    if self.B is not None:
+      kd = pl.terms[0].b_dims[0]
+      origin, stride = self.B.origin[kd], self.B.strides[kd]
+    else:
+      origin, stride = pl.L.origin[0], pl.L.strides[0]
+    for j in range(self.KB):
+      k = kb * self.KB + j
       if k < self.K:
-        w(f'{dst}[{j}] = lq[{(k + pl.klo - L.origin[0]) * L.strides[0]}];')
+        w(f'{dst}[{j}] = lq[{(k + pl.klo - origin) * stride}];')
       else:
         w(f'{dst}[{j}] = 0.0;')
 
@@ -340,6 +421,11 @@ class _Mfma:
     or split into B tile ``tile``."""
     w = self.w
     kb, ti = step
+    cast = '' if self.mode == 'fp64' else '(float)'
+    if self.B is not None:
+      for j in js:
+        w(f'tf{tile}[gcol * {pitch} + {j}] = {cast}{q}[{j}];')
+      return
     tabs = self.tables[ti]
     for j in js:
       v = f'tv{ti}_{j}'
@@ -348,13 +434,13 @@ class _Mfma:
       for si, (src, pv) in enumerate(tabs):
         w(f'  {v} = __builtin_fma(__hiloint2double(__builtin_amdgcn_ds_bpermute({src}, qhi), '
           f'__builtin_amdgcn_ds_bpermute({src}, qlo)), w{ti}_{si}, {v});')
-      w(f'  tf{tile}[gcol * {pitch} + {j}] = (float){v}; }}')
+      w(f'  tf{tile}[gcol * {pitch} + {j}] = {cast}{v}; }}')
 
   def mfma_list(self, kb, ti, tile, pitch):
     """Source lines of step (kb, ti)'s MFMAs from B tile ``tile``: fragment loads, then the
     MFMAs, as statements of the enclosing scope (names are unique per step)."""
     RT, CT = self.RT, self.CT
-    nks = KB // self.kstep
+    nks = self.KB // self.kstep
     out = []
     sfx = f'_{kb}_{ti}'
     for ks in range(nks):
@@ -366,8 +452,8 @@ class _Mfma:
           out.append(f'tf_split3(*reinterpret_cast<const float4*>(tf{tile} + {o}), bh{b}, bm{b}, '
                      f'bl{b});')
         else:
-          out.append(f'const float bv{ks}_{ct}{sfx} = tf{tile}[({ct * 16} + lane % 16) * {pitch} + '
-                     f'{ks * 4} + lane / 16];')
+          out.append(f'const {self.real} bv{ks}_{ct}{sfx} = tf{tile}[({ct * 16} + lane % 16) * '
+                     f'{pitch} + {ks * 4} + lane / 16];')
     for ks in range(nks):
       for rt in range(RT):
         frag = f'{((ti * self.KS + kb * nks + ks) * RT + rt) * 64} + lane'
@@ -376,7 +462,7 @@ class _Mfma:
           out.append(f'tf_s4 ah{a}, am{a}, al{a};')
           out.append(f'tf_split3(kf4[{frag}], ah{a}, am{a}, al{a});')
         else:
-          out.append(f'const float av{ks}_{rt}{sfx} = kf[{frag}];')
+          out.append(f'const {self.real} av{ks}_{rt}{sfx} = kf[{frag}];')
         for ct in range(CT):
           a = f'acc[{rt * CT + ct}]'
           if self.mode == 'bf16x3':
@@ -388,9 +474,50 @@ class _Mfma:
             for x, y in (('am', 'bm'), ('al', 'bh'), ('ah', 'bl'), ('am', 'bh'), ('ah', 'bm')):
               out.append(f'{r} = {mf}({x}{ai}, {y}{bi}, {r}, 0, 0, 0);')
           else:
-            out.append(f'{a} = __builtin_amdgcn_mfma_f32_16x16x4f32(av{ks}_{rt}{sfx}, '
-                       f'bv{ks}_{ct}{sfx}, {a}, 0, 0, 0);')
+            mf = '__builtin_amdgcn_mfma_f64_16x16x4f64' if self.mode == 'fp64' else \
+                '__builtin_amdgcn_mfma_f32_16x16x4f32'
+            out.append(f'{a} = {mf}(av{ks}_{rt}{sfx}, bv{ks}_{ct}{sfx}, {a}, 0, 0, 0);')
     return out
+
+  def dest_tile_loop(self, body):
+    """Loops over the lane's accumulator entries (tile, entry jj) that hold a destination entry
+    of an element in the batch; ``body(d)`` writes the statements for entry ``acc[i][jj]``
+    at ``d`` + ``off``."""
+    w = self.w
+    pl = self.pl
+    q, G, RT, CT = self.q, self.G, self.RT, self.CT
+    dp = pl.dest_product
+    o0, o1 = dp.dest_origin
+    s0, s1 = dp.dest_strides
+    d = pl.dest
+    w('#pragma unroll')
+    with w.block(f'for (int rt = 0; rt < {RT}; ++rt)'):
+      w('#pragma unroll')
+      with w.block(f'for (int ct = 0; ct < {CT}; ++ct)'):
+        w('const int g = ct * 16 + lane % 16;')
+        w(f'const int so = g / {q};')
+        w(f'const int co = g % {q};')
+        w('const size_t eo = e0 + so;')
+        w(f'if (g >= {G * q} || eo >= numElements) continue;')
+        body(d, o0, o1, s0, s1)
+
+  def load_destination(self):
+    """FP64 accumulation into the destination: its old values start the accumulators, loaded
+    before the contraction so that their latency overlaps it (the MFMA D layout: entry jj of
+    lane l is row 4 jj + l / 16 of the tile, column l % 16)."""
+    w = self.w
+    pl = self.pl
+
+    def body(d, o0, o1, s0, s1):
+      w(f'const {d.dtype}* __restrict__ src = {d.name} + eo * {d.element_stride}ull + '
+        f'{d.name}_extraOffset;')
+      w('#pragma unroll')
+      with w.block('for (int jj = 0; jj < 4; ++jj)'):
+        w('const int row = rt * 16 + 4 * jj + lane / 16;')
+        w(f'if (row >= {self.rows}) continue;')
+        w(f'acc[rt * {self.CT} + ct][jj] = src[(size_t)(row + {pl.rows[0] - o0}) * {s0} + '
+          f'(size_t)(co + {pl.cols[0] - o1}) * {s1}];')
+    self.dest_tile_loop(body)
 
   def store(self):
     w = self.w
@@ -414,7 +541,12 @@ class _Mfma:
           f'{d.name}_extraOffset;')
         w('#pragma unroll')
         with w.block('for (int jj = 0; jj < 4; ++jj)'):
-          w('const int row = rt * 16 + 4 * (lane / 16) + jj;')
+          # accumulator entry jj of the lane: rows 4 (lane / 16) + jj of the tile for the FP32
+          # instruction (four consecutive rows per lane), rows 4 jj + lane / 16 for the FP64 one
+          if self.mode == 'fp64':
+            w('const int row = rt * 16 + 4 * jj + lane / 16;')
+          else:
+            w('const int row = rt * 16 + 4 * (lane / 16) + jj;')
           w(f'if (row >= {self.rows}) continue;')
           w(f'const size_t off = (size_t)(row + {pl.rows[0] - o0}) * {s0} + '
             f'(size_t)(co + {pl.cols[0] - o1}) * {s1};')
@@ -422,19 +554,19 @@ class _Mfma:
             w(f'double v = (double)acc[rt * {CT} + ct][jj] + (double)acr[rt * {CT} + ct][jj];')
           else:
             w(f'double v = (double)acc[rt * {CT} + ct][jj];')
-          if pl.init == 'load':
+          if pl.init == 'load' and not self.preload:
             w('v += dst[off];')
           w('dst[off] = v;')
 
   def grid_setup(self, lds_bytes, threads):
-    return persistent_grid(self.kname, lds_bytes, threads, self.G * WAVES,
+    return persistent_grid(self.kname, lds_bytes, threads, self.G * self.waves,
                            f'launcher_{self.base_name}')
 
 
 def emit(program, context, base_name, mode):
   from . import stream
   pl, why = stream.plan(program)
-  reason = unsupported_reason(program, pl) if pl is not None else why
+  reason = unsupported_reason(program, pl, mode) if pl is not None else why
   if reason is not None:
     return None, reason
   return _Mfma(program, context, base_name, pl, mode).emit(), None

@@ -186,7 +186,7 @@ class Generator:
         raise GenerationError(f'{self._emitter_choice} does not fuse pointwise statements')
     choice = self._emitter_choice
     emitted = None
-    if choice in ('mfma_fp32', 'mfma_bf16x3'):
+    if choice in ('mfma_fp64', 'mfma_fp32', 'mfma_bf16x3'):
       from ..emit import mfma
       emitted, reason = mfma.emit(self._program, self._context, self._base_kernel_name,
                                   choice[len('mfma_'):])
@@ -207,7 +207,7 @@ class Generator:
       elif choice == 'tuned':
         raise GenerationError(f'tuned emitter cannot generate this chain: {reason}')
     if emitted is None:
-      if choice not in ('auto', 'tuned', 'plain', 'mfma_fp32', 'mfma_bf16x3'):
+      if choice not in ('auto', 'tuned', 'plain', 'mfma_fp64', 'mfma_fp32', 'mfma_bf16x3'):
         raise ValueError(f'unknown emitter {choice!r}')
       emitted = plain.emit(self._program, self._context, self._base_kernel_name,
                            self._num_threads, self._thread_block_policy_type)

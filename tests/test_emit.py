@@ -115,10 +115,38 @@ def test_matrix_core_forms_refuse_register_stage_chains():
   from tensorforge_amd.common.exceptions import GenerationError
   for make in (W.flux_local, W.flux_neigh):
     w = make()
-    for mode in ('mfma_fp32', 'mfma_bf16x3'):
+    for mode in ('mfma_fp64', 'mfma_fp32', 'mfma_bf16x3'):
       g = Generator(w.descrs, Context('gfx950', 'hip', FloatingPointType.DOUBLE), emitter=mode)
       with pytest.raises(GenerationError, match='register stages'):
         g.generate()
+
+
+def test_fp64_matrix_core_form_shapes():
+  """The FP64 matrix-core form: the GEMM runs eight wavefronts with 8-row blocks, the dense
+  order-6 chain four wavefronts (8-row blocks: 16 would not fit in LDS as doubles); order 7 does
+  not fit, and sparse shared operands and FP32 chains are refused."""
+  import pytest
+  from tensorforge_amd.common.exceptions import GenerationError
+  ctx = Context('gfx950', 'hip', FloatingPointType.DOUBLE)
+  g = Generator(W.gemm().descrs, ctx, emitter='mfma_fp64')
+  g.generate()
+  sch = g.get_emitted().schedule
+  assert (sch['kind'], sch['waves_per_block'], sch['k_block']) == ('mfma_fp64', 8, 8)
+  assert sch['lds_bytes'] <= 160 * 1024
+  assert '__builtin_amdgcn_mfma_f64_16x16x4f64' in g.get_kernel()
+  g = Generator(W.ader_volume(sparse_k=False).descrs, ctx, emitter='mfma_fp64')
+  g.generate()
+  sch = g.get_emitted().schedule
+  assert (sch['waves_per_block'], sch['k_block']) == (4, 8) and sch['lds_bytes'] <= 160 * 1024
+  for w, why in ((W.ader_volume(b=84, sparse_k=False), 'do not fit in LDS'),
+                 (W.ader_volume(), 'sparse shared operand')):
+    g = Generator(w.descrs, ctx, emitter='mfma_fp64')
+    with pytest.raises(GenerationError, match=why):
+      g.generate()
+  g = Generator(W.gemm(FloatingPointType.FLOAT).descrs,
+                Context('gfx950', 'hip', FloatingPointType.FLOAT), emitter='mfma_fp64')
+  with pytest.raises(GenerationError, match='FP64 chains'):
+    g.generate()
 
 
 def test_forms_of_one_chain_export_distinct_kernel_symbols():
