@@ -381,6 +381,19 @@ def gemm_leg(device, steps, warmup, cpu_seconds, with_cpu, n=65536):
                    'max_normwise_rel_err_vs_closed_form': float(
                      (np.abs(ref - want[:m]).reshape(m, -1).max(axis=1) /
                       np.abs(want[:m]).reshape(m, -1).max(axis=1)).max())}
+    # this package's omptarget launcher of the same GEMM (Context(backend='omptarget')), over
+    # the whole batch
+    _, gen, omp_fn = catalog.load_omp('gemm_f64')
+    C = host['C'].copy()
+    args, _keep = catalog.omp_arguments(gen, {'A': host['A'], 'B': host['B'], 'C': C}, n)
+    t0 = time.perf_counter()
+    omp_fn(*args)
+    t = time.perf_counter() - t0
+    cpu['generated_omptarget'] = {
+      'value': n / t, 'elements': n, 'seconds': t,
+      'max_normwise_rel_err_vs_closed_form': float(
+        (np.abs(C.reshape(n, q, b) - want).reshape(n, -1).max(axis=1) /
+         np.abs(want).reshape(n, -1).max(axis=1)).max())}
     res['cpu_baseline'] = cpu
   return res
 
