@@ -127,3 +127,34 @@ def test_timing_database_size_and_sweep_record():
   assert all(r['runs'] == 2 for r in recs)
   assert meta['drawn'] >= meta['kept'] >= meta['timed_chains']
   assert sum(p['chains'] for p in meta['measure_passes']) == 2 * meta['timed_chains']
+
+
+def test_autotune_chain_kind_and_candidate_records():
+  """The autotuner's chain family (a feature) and its candidate records, without a GPU: the
+  volume chain is 'fused', a GEMM 'linear', a 3-index contraction 'three_index'; every record
+  carries the full feature vector; unknown methods are refused."""
+  import pytest
+  from tensorforge_amd import workloads as W
+  from tensorforge_amd.common.basic_types import FloatingPointType
+  from tensorforge_amd.tune import autotune as at
+  from tensorforge_amd.tune import features as F
+  from tensorforge_amd.tune.sweep import chain_descriptions
+  from tensorforge_amd.generators.generator import Generator
+  from tensorforge_amd.common.context import Context
+
+  def kind(descrs, fp):
+    g = Generator(descrs, Context('gfx950', 'hip', fp))
+    g.generate()
+    return at.chain_kind(g)
+  assert kind(W.ader_volume().descrs, FloatingPointType.DOUBLE) == 'fused'
+  assert kind(W.gemm().descrs, FloatingPointType.DOUBLE) == 'linear'
+  i = next(i for i in range(100) if chain_descriptions(7, i)[3]['kind'] == 'three_index')
+  descrs, fp, batch, info = chain_descriptions(7, i)
+  assert kind(descrs, fp) == 'three_index'
+  w = W.gemm()
+  cands, recs = at.candidates(w.descrs, w.fp_type, 65536, 'attest')
+  assert len(cands) == len(recs) >= 2
+  assert all(len(r['features']) == len(F.NAMES) for r in recs)
+  assert len({g.get_emitted().kernel_name for _, g in cands}) == len(cands)
+  with pytest.raises(ValueError):
+    at.autotune(w.descrs, w.fp_type, 65536, method='fastest')
