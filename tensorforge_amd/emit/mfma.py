@@ -44,15 +44,17 @@ KB = 16            # contraction rows per block
 LDS_LIMIT = 160 * 1024
 
 
-def _shape(pl, mode):
+def _shape(pl, mode, sched=None):
   """(wavefronts per workgroup, contraction rows per block). The FP32 and BF16x3 forms: 4 and 16.
-  FP64 with a per-element right operand (the GEMM): eight wavefronts (two per SIMD, so one's
-  memory waits overlap the other's MFMAs; 224 registers) with blocks of 8 rows, whose B tiles are
-  half as large. FP64 chains that form temporaries hold their weights and bpermute sources as
-  well and keep four wavefronts (two would spill), with 16-row blocks when they fit in LDS."""
+  FP64: four wavefronts (one per SIMD: the registers hold the accumulators, the destination's
+  old values loaded at the start of each group and the streamed rows of three blocks), with
+  16-row blocks, or 8 when the shared operands as doubles and 16-row B tiles do not fit in LDS.
+  ``sched.mfma_waves`` and ``sched.mfma_k_block`` override the choice."""
   if mode != 'fp64':
     return WAVES, KB
-  shapes = ((8, 8), (4, 16), (4, 8)) if pl.terms[0].kind == 'global' else ((4, 16), (4, 8))
+  if sched is not None and sched.mfma_waves and sched.mfma_k_block:
+    return sched.mfma_waves, sched.mfma_k_block
+  shapes = ((4, 16), (4, 8))
   for waves, kb in shapes:
     if _lds_bytes(pl, mode, kb, waves) <= LDS_LIMIT:
       return waves, kb
@@ -72,7 +74,7 @@ def _lds_bytes(pl, mode, kb, waves=WAVES):
   return frag * 4 + waves * 2 * 64 * 17 * 4
 
 
-def unsupported_reason(program, pl, mode='fp32'):
+def unsupported_reason(program, pl, mode='fp32', sched=None):
   if pl is None:
     return 'not a streaming chain'
   if program.fp_type != FloatingPointType.DOUBLE:
@@ -110,13 +112,19 @@ def unsupported_reason(program, pl, mode='fp32'):
     return 'a per-element operand is not strided'
   if program.epilogue is not None:
     return 'pointwise statements fused into the chain'
-  if _lds_bytes(pl, mode, _shape(pl, mode)[1], _shape(pl, mode)[0]) > LDS_LIMIT:
+  waves, kb = _shape(pl, mode, sched)
+  if waves not in (4, 8) or kb not in ((8, 16) if mode == 'fp64' else (KB,)):
+    return f'{waves} wavefronts with {kb}-row blocks'
+  if _lds_bytes(pl, mode, kb, waves) > LDS_LIMIT:
     return 'shared operands do not fit in LDS'
+  if sched is not None and sched.mfma_dest not in ('', 'acc', 'regs', 'end') or \
+      (mode != 'fp64' and sched is not None and sched.mfma_dest not in ('', 'end')):
+    return f'destination mode {sched.mfma_dest!r}'
   return None
 
 
 class _Mfma:
-  def __init__(self, program, context, base_name, pl, mode):
+  def __init__(self, program, context, base_name, pl, mode, sched=None):
     self.prog, self.ctx, self.pl, self.mode = program, context, pl, mode
     self.base_name = base_name
     self.kname = f'kernel_{base_name}'
@@ -128,7 +136,7 @@ class _Mfma:
     self.rows = pl.rows[1] - pl.rows[0]
     self.RT = (self.rows + 15) // 16
     self.K = pl.khi - pl.klo
-    self.waves, self.KB = _shape(pl, mode)
+    self.waves, self.KB = _shape(pl, mode, sched)
     self.NKB = (self.K + self.KB - 1) // self.KB
     # k per MFMA: 16x16x4 f64 and f32, 16x16x16 bf16
     self.kstep = 16 if mode == 'bf16x3' else 4
@@ -138,8 +146,13 @@ class _Mfma:
     # one product with a per-element right operand: its rows go to the B tile as they are
     self.B = pl.terms[0].B if pl.terms[0].kind == 'global' else None
     self.real = 'double' if mode == 'fp64' else 'float'
-    # FP64 accumulates into the destination's old values (loaded at the start of the group)
-    self.preload = mode == 'fp64' and pl.init == 'load'
+    # where an accumulating product's old destination values wait: 'acc' (FP64 default: the
+    # accumulators start from them, loaded at the start of the group), 'regs' (registers of their
+    # own, loaded after the group's first streamed block, added at the end), 'end' (loaded and
+    # added at the end; the FP32 and BF16x3 forms)
+    dest = (sched.mfma_dest if sched is not None else '') or ('regs' if mode == 'fp64' else 'end')
+    self.dest_mode = dest if pl.init == 'load' else 'end'
+    self.preload = self.dest_mode == 'acc'
 
   # --------------------------------------------------------------- per-lane star slots
   def slot_tables(self, t):
@@ -337,12 +350,8 @@ class _Mfma:
       ty = f'{"const " if a.const else ""}{a.dtype}'
       w(f'{ty}* __restrict__ g_{a.name} = {a.name} + eSafe * {a.element_stride}ull + '
         f'{a.name}_extraOffset;')
-    if self.B is not None:
-      # the lane's column of the per-element right operand
-      B = self.B
-      kd, nd = pl.terms[0].b_dims
-      w(f'const double* __restrict__ lq = g_{B.arg.name} + (col + {pl.cols[0] - B.origin[nd]}) * '
-        f'{B.strides[nd]};')
+    if self.mode == 'fp64':
+      w(f'const double* __restrict__ lq = {self.column_ptr("eSafe")};')
     else:
       w(f'const double* __restrict__ lq = g_{L.arg.name} + (col + {pl.l_cols[0] - L.origin[1]}) '
         f'* {L.strides[1]};')
@@ -372,6 +381,8 @@ class _Mfma:
     KBr = self.KB
     w(f'double qa[{KBr}], qb[{KBr}];')
     self.load_rows(0, 'qa')
+    if self.dest_mode == 'regs':
+      self.load_destination('cpre')
     self.t_rows(steps[0], 'qa', 0, pitch, range(KBr))
     self.tile_fence()
     for si, (kb, t) in enumerate(steps):
@@ -395,7 +406,22 @@ class _Mfma:
       self.tile_fence()
     self.store()
 
-  def load_rows(self, kb, dst):
+  def column_ptr(self, esafe):
+    """The lane's column of the streamed per-element operand (the right operand of a GEMM, or
+    the operand the temporaries are formed from) for element ``esafe``."""
+    pl = self.pl
+    if self.B is not None:
+      X, nd = self.B, pl.terms[0].b_dims[1]
+      c0 = pl.cols[0]
+    else:
+      X, nd = pl.L, 1
+      c0 = pl.l_cols[0]
+    a = X.arg
+    return (f'{a.name} + {esafe} * {a.element_stride}ull + {a.name}_extraOffset + '
+            f'(col + {c0 - X.origin[nd]}) * {X.strides[nd]}')
+
+  def load_rows(self, kb, dst, ptr='lq'):
+    """Rows of contraction block ``kb`` of the lane's column (at ``ptr``) into ``dst``."""
     w = self.w
     pl = self.pl
     if self.B is not None:
@@ -406,7 +432,7 @@ class _Mfma:
     for j in range(self.KB):
       k = kb * self.KB + j
       if k < self.K:
-        w(f'{dst}[{j}] = lq[{(k + pl.klo - origin) * stride}];')
+        w(f'{dst}[{j}] = {ptr}[{(k + pl.klo - origin) * stride}];')
       else:
         w(f'{dst}[{j}] = 0.0;')
 
@@ -501,12 +527,17 @@ class _Mfma:
         w(f'if (g >= {G * q} || eo >= numElements) continue;')
         body(d, o0, o1, s0, s1)
 
-  def load_destination(self):
-    """FP64 accumulation into the destination: its old values start the accumulators, loaded
-    before the contraction so that their latency overlaps it (the MFMA D layout: entry jj of
-    lane l is row 4 jj + l / 16 of the tile, column l % 16)."""
+  def load_destination(self, into='acc'):
+    """FP64 accumulation into the destination: its old values, loaded before the contraction so
+    that their latency overlaps it, into the accumulators or into registers ``into`` of their own
+    (the MFMA D layout: entry jj of lane l is row 4 jj + l / 16 of the tile, column l % 16)."""
     w = self.w
     pl = self.pl
+    if into != 'acc':
+      w(f'tf_d4 {into}[{self.RT * self.CT}];')
+      w('#pragma unroll')
+      with w.block(f'for (int i = 0; i < {self.RT * self.CT}; ++i)'):
+        w(f'{into}[i] = tf_d4{{0.0, 0.0, 0.0, 0.0}};')
 
     def body(d, o0, o1, s0, s1):
       w(f'const {d.dtype}* __restrict__ src = {d.name} + eo * {d.element_stride}ull + '
@@ -515,7 +546,7 @@ class _Mfma:
       with w.block('for (int jj = 0; jj < 4; ++jj)'):
         w('const int row = rt * 16 + 4 * jj + lane / 16;')
         w(f'if (row >= {self.rows}) continue;')
-        w(f'acc[rt * {self.CT} + ct][jj] = src[(size_t)(row + {pl.rows[0] - o0}) * {s0} + '
+        w(f'{into}[rt * {self.CT} + ct][jj] = src[(size_t)(row + {pl.rows[0] - o0}) * {s0} + '
           f'(size_t)(co + {pl.cols[0] - o1}) * {s1}];')
     self.dest_tile_loop(body)
 
@@ -554,8 +585,10 @@ class _Mfma:
             w(f'double v = (double)acc[rt * {CT} + ct][jj] + (double)acr[rt * {CT} + ct][jj];')
           else:
             w(f'double v = (double)acc[rt * {CT} + ct][jj];')
-          if pl.init == 'load' and not self.preload:
+          if self.dest_mode == 'end' and pl.init == 'load':
             w('v += dst[off];')
+          elif self.dest_mode == 'regs':
+            w(f'v += cpre[rt * {CT} + ct][jj];')
           w('dst[off] = v;')
 
   def grid_setup(self, lds_bytes, threads):
@@ -563,10 +596,10 @@ class _Mfma:
                            f'launcher_{self.base_name}')
 
 
-def emit(program, context, base_name, mode):
+def emit(program, context, base_name, mode, sched=None):
   from . import stream
   pl, why = stream.plan(program)
-  reason = unsupported_reason(program, pl, mode) if pl is not None else why
+  reason = unsupported_reason(program, pl, mode, sched) if pl is not None else why
   if reason is not None:
     return None, reason
-  return _Mfma(program, context, base_name, pl, mode).emit(), None
+  return _Mfma(program, context, base_name, pl, mode, sched).emit(), None

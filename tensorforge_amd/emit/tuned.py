@@ -99,6 +99,13 @@ class Schedule:
   stream_dma_slots: int = 3
   # how the DPP broadcast is issued: 'asm' (one v_fmac_*_dpp) or 'builtin' (v_mov_*_dpp + fma)
   dpp: str = 'asm'
+  # FP64 matrix-core form: wavefronts per workgroup and contraction rows per block (0: chosen
+  # from the chain), and where the destination's old values wait for the accumulation:
+  # 'acc' (loaded into the accumulators before the first MFMA), 'regs' (loaded into registers of
+  # their own at the start of the group, added at the end), 'end' (loaded at the end), '' (chosen)
+  mfma_waves: int = 0
+  mfma_k_block: int = 0
+  mfma_dest: str = ''
 
   def key(self):
     return tuple(sorted(asdict(self).items()))

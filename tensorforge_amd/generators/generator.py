@@ -189,7 +189,7 @@ class Generator:
     if choice in ('mfma_fp64', 'mfma_fp32', 'mfma_bf16x3'):
       from ..emit import mfma
       emitted, reason = mfma.emit(self._program, self._context, self._base_kernel_name,
-                                  choice[len('mfma_'):])
+                                  choice[len('mfma_'):], self._schedule)
       if emitted is None:
         raise GenerationError(f'{choice} cannot generate this chain: {reason}')
     if choice in ('auto', 'tuned'):

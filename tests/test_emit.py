@@ -122,18 +122,31 @@ def test_matrix_core_forms_refuse_register_stage_chains():
 
 
 def test_fp64_matrix_core_form_shapes():
-  """The FP64 matrix-core form: the GEMM runs eight wavefronts with 8-row blocks, the dense
-  order-6 chain four wavefronts (8-row blocks: 16 would not fit in LDS as doubles); order 7 does
-  not fit, and sparse shared operands and FP32 chains are refused."""
+  """The FP64 matrix-core form: the GEMM runs four wavefronts with 16-row blocks and the
+  destination's old values in registers of their own, the dense order-6 chain four wavefronts
+  with 8-row blocks (16 would not fit in LDS as doubles); schedule knobs override the shape and
+  the destination mode; order 7 does not fit, and sparse shared operands, FP32 chains and shapes
+  outside the form are refused."""
   import pytest
   from tensorforge_amd.common.exceptions import GenerationError
   ctx = Context('gfx950', 'hip', FloatingPointType.DOUBLE)
   g = Generator(W.gemm().descrs, ctx, emitter='mfma_fp64')
   g.generate()
   sch = g.get_emitted().schedule
-  assert (sch['kind'], sch['waves_per_block'], sch['k_block']) == ('mfma_fp64', 8, 8)
+  assert (sch['kind'], sch['waves_per_block'], sch['k_block']) == ('mfma_fp64', 4, 16)
   assert sch['lds_bytes'] <= 160 * 1024
   assert '__builtin_amdgcn_mfma_f64_16x16x4f64' in g.get_kernel()
+  assert 'v += cpre[' in g.get_kernel()
+  from tensorforge_amd.emit.tuned import Schedule
+  g = Generator(W.gemm().descrs, ctx, emitter='mfma_fp64',
+                schedule=Schedule(mfma_waves=8, mfma_k_block=8, mfma_dest='acc'))
+  g.generate()
+  sch = g.get_emitted().schedule
+  assert (sch['waves_per_block'], sch['k_block']) == (8, 8) and 'cpre' not in g.get_kernel()
+  for bad in (dict(mfma_waves=2, mfma_k_block=8), dict(mfma_dest='later')):
+    g = Generator(W.gemm().descrs, ctx, emitter='mfma_fp64', schedule=Schedule(**bad))
+    with pytest.raises(GenerationError):
+      g.generate()
   g = Generator(W.ader_volume(sparse_k=False).descrs, ctx, emitter='mfma_fp64')
   g.generate()
   sch = g.get_emitted().schedule
