@@ -1,4 +1,4 @@
-"""Schedule choice for one description list, the way the sweep judges it.
+"""Schedule choice for one description list: the cost model's pick, a pruned search or the sweep.
 
 ``autotune(descrs, fp_type, batch, method)`` generates every candidate form the chain admits
 (``space.applicable``), then:
