@@ -158,3 +158,32 @@ def test_autotune_chain_kind_and_candidate_records():
   assert len({g.get_emitted().kernel_name for _, g in cands}) == len(cands)
   with pytest.raises(ValueError):
     at.autotune(w.descrs, w.fp_type, 65536, method='fastest')
+
+
+def test_sweep_pack_averages_passes_and_counts(tmp_path):
+  """``sweep.pack``: a candidate timed in two passes keeps the mean of its times and ``runs`` 2,
+  features and times are rounded, and the sweep's record sums the directories' counts."""
+  import gzip
+  import json
+  from tensorforge_amd.tune import model as M
+  from tensorforge_amd.tune.sweep import pack
+  d = tmp_path / 'sw'
+  d.mkdir()
+  json.dump({'chains': [{'chain': 3}, {'chain': 4}], 'skipped': 5, 'generate_seconds': 2.0,
+             'compile_seconds': 8.0}, open(d / 'index.json', 'w'))
+  rec = lambda c, lab, t: {'chain': c, 'label': lab, 'time': t, 'batch': 1000, 'info': {},
+                           'features': [1.234567891, 2.0]}
+  f1, f2 = tmp_path / 'a.jsonl', tmp_path / 'b.jsonl'
+  f1.write_text('\n'.join(json.dumps(r) for r in (rec(3, 'plain', 2e-3), rec(3, 'x', 1e-3))))
+  f2.write_text('\n'.join(json.dumps(r) for r in (rec(3, 'plain', 4e-3), rec(4, 'x', 5e-4))))
+  out = str(tmp_path / 'db.jsonl.gz')
+  meta = pack([str(d)], [str(f1), str(f2)], out, 'test')
+  recs = M.load(out)
+  by = {(r['chain'], r['label']): r for r in recs}
+  assert by[(3, 'plain')]['time'] == 3e-3 and by[(3, 'plain')]['runs'] == 2
+  assert by[(3, 'x')]['runs'] == 1 and by[(4, 'x')]['time'] == 5e-4
+  assert by[(3, 'x')]['features'][0] == 1.234568 and all(r['set'] == 'test' for r in recs)
+  assert meta == {'drawn': 7, 'kept': 2, 'generate_cpu_seconds': 2.0, 'compile_cpu_seconds': 8.0,
+                  'timed_chains': 2, 'timed_candidates': 3}
+  with gzip.open(out, 'rt') as f:
+    assert len(f.read().splitlines()) == 3
