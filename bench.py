@@ -446,11 +446,11 @@ def tuner_leg():
   MI355X); the networks train on the GPU in the project's HIP kernels. Also the sweep's record:
   chains drawn and kept, the build's CPU seconds and the measure passes' GPU seconds."""
   from tensorforge_amd.tune import model as M
+  from tensorforge_amd.tune.autotune import SETTINGS
   data = os.path.join(ROOT, 'tensorforge_amd', 'tune', 'data')
   recs = M.load(os.path.join(data, 'timings_mi355x.jsonl.gz'))
   t0 = time.time()
-  s = M.cross_validate(recs, folds=5, seed=0, prune_k=3, epochs=150, batch=2048, lr=0.004,
-                       ensemble=8, target='gap', cap=0.25)
+  s = M.cross_validate(recs, folds=5, seed=0, prune_k=3, **SETTINGS)
   s['cross_validation_seconds'] = time.time() - t0
   s['candidates_per_chain'] = len(recs) / s['chains']
   with open(os.path.join(data, 'timings_mi355x.json')) as f:

@@ -23,12 +23,11 @@ DATA = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                     'tune', 'data')
 DB = os.path.join(DATA, 'timings_mi355x.jsonl.gz')
 
-# the settings the bench's cost-model leg uses
-SETTINGS = dict(epochs=150, batch=2048, lr=0.004, ensemble=8, target='gap', cap=0.25)
 
 
 def test_single_pick_within_5pct_on_90pct_of_held_out_chains(device):
   from tensorforge_amd.tune import model as M
+  from tensorforge_amd.tune.autotune import SETTINGS       # the settings the bench leg uses
   recs = M.load(DB)
   s = M.cross_validate(recs, folds=5, seed=0, prune_k=3, **SETTINGS)
   print(s)
