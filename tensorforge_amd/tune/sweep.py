@@ -86,14 +86,19 @@ def chain_descriptions(seed_chain, i):
 
 
 def build(directory, count, seed, jobs=8, max_source=MAX_SOURCE, precision='exact', start=0):
+  """Draw and compile chains ``start .. start + count - 1`` into ``directory``; resumable: the
+  index records the chains kept and the chains skipped (fewer than two candidates), and a
+  later call draws only the others."""
   os.makedirs(directory, exist_ok=True)
   index_path = os.path.join(directory, 'index.json')
-  entries, skipped, gen_s, cc_s = [], 0, 0.0, 0.0
+  entries, skipped_set, gen_s, cc_s = [], set(), 0.0, 0.0
   if os.path.exists(index_path):
     old = json.load(open(index_path))
-    entries, skipped = old['chains'], old.get('skipped', 0)
+    # an index without the skipped chains' list (an older format) cannot tell which indices
+    # were skipped: they are drawn again, and counted once
+    entries, skipped_set = old['chains'], set(old.get('skipped_chains', []))
     gen_s, cc_s = old.get('generate_seconds', 0.0), old.get('compile_seconds', 0.0)
-  done = {e['chain'] for e in entries}
+  done = {e['chain'] for e in entries} | skipped_set
   todo = [(i, seed, directory, max_source, precision) for i in range(start, start + count)
           if i not in done]
   t0 = time.time()
@@ -102,7 +107,8 @@ def build(directory, count, seed, jobs=8, max_source=MAX_SOURCE, precision='exac
     with open(index_path + '.tmp', 'w') as f:
       json.dump({'seed': seed, 'feature_names': F.NAMES, 'max_source': max_source,
                  'precision': precision, 'chains': sorted(entries, key=lambda e: e['chain']),
-                 'skipped': skipped, 'generate_seconds': gen_s, 'compile_seconds': cc_s}, f)
+                 'skipped': len(skipped_set), 'skipped_chains': sorted(skipped_set),
+                 'generate_seconds': gen_s, 'compile_seconds': cc_s}, f)
     os.replace(index_path + '.tmp', index_path)
 
   with concurrent.futures.ProcessPoolExecutor(jobs) as pool:
@@ -112,7 +118,7 @@ def build(directory, count, seed, jobs=8, max_source=MAX_SOURCE, precision='exac
       if rec['candidates']:
         entries.append(rec)
       else:
-        skipped += 1
+        skipped_set.add(rec['chain'])
       if k % 20 == 19 or k + 1 == len(todo):
         save()
         print(f'{k + 1}/{len(todo)} chains drawn, {len(entries)} kept '

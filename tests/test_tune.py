@@ -187,3 +187,33 @@ def test_sweep_pack_averages_passes_and_counts(tmp_path):
                   'timed_chains': 2, 'timed_candidates': 3}
   with gzip.open(out, 'rt') as f:
     assert len(f.read().splitlines()) == 3
+
+
+def test_sweep_build_resumes_without_redrawing(tmp_path, monkeypatch):
+  """``sweep.build`` records the chains it kept and skipped, and a resumed build draws only the
+  indices it has not seen (each chain counted once)."""
+  import json
+  from tensorforge_amd.tune import sweep
+  drawn = []
+
+  def fake_job(args):
+    i = args[0]
+    drawn.append(i)
+    rec = {'chain': i, 'info': {}, 'batch': 1000, 'dropped_large': [], 'candidates': []}
+    if i % 3 == 0:
+      rec['candidates'] = [{'label': 'plain'}, {'label': 'x'}]
+      rec['so'] = f'libc{i}.so'
+    return rec, 0.5, 1.0 if rec['candidates'] else 0.0
+
+  monkeypatch.setattr(sweep, '_chain_job', fake_job)
+  monkeypatch.setattr(sweep.concurrent.futures, 'ProcessPoolExecutor',
+                      sweep.concurrent.futures.ThreadPoolExecutor)
+  d = str(tmp_path / 'sw')
+  sweep.build(d, 10, seed=1, jobs=2, start=100)
+  sweep.build(d, 12, seed=1, jobs=2, start=100)
+  assert sorted(drawn) == list(range(100, 112))
+  idx = json.load(open(tmp_path / 'sw' / 'index.json'))
+  kept = [c['chain'] for c in idx['chains']]
+  assert kept == [102, 105, 108, 111]
+  assert idx['skipped'] == len(idx['skipped_chains']) == 8
+  assert idx['generate_seconds'] == 6.0 and idx['compile_seconds'] == 4.0
