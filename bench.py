@@ -442,9 +442,10 @@ def pointwise_leg(device, steps, warmup, n=262144):
 
 def tuner_leg():
   """Cost model vs the exhaustive sweep: five-fold cross-validation by chain over the
-  committed timing database (every applicable candidate of 2,191 random chains timed twice on
-  MI355X); the networks train on the GPU in the project's HIP kernels. Also the sweep's record:
-  chains drawn and kept, the build's CPU seconds and the measure passes' GPU seconds."""
+  committed timing database (20,000 random chains drawn; every applicable candidate of the 7,921
+  that admit a choice timed twice on MI355X); the networks train on the GPU in the project's HIP
+  kernels. Also the sweep's record: chains drawn and kept, the build's CPU seconds and the
+  measure passes' GPU seconds."""
   from tensorforge_amd.tune import model as M
   from tensorforge_amd.tune.autotune import SETTINGS
   data = os.path.join(ROOT, 'tensorforge_amd', 'tune', 'data')

@@ -1,12 +1,12 @@
 """The learned cost model against the exhaustive sweep (GPU: the network trains in HIP).
 
-Data: ``tensorforge_amd/tune/data/timings_mi355x.jsonl.gz``, every applicable candidate of 2,191
-random contraction chains (2 to 5 products, seed 2026; of 6,080 chains drawn, the others admit
-fewer than two candidate forms) timed on an MI355X in two measure passes whose per-candidate
-medians are averaged (``python -m tensorforge_amd.tune.sweep build | measure | pack``; the
-sweep's record, ``timings_mi355x.json``, holds the build's CPU seconds and each pass's GPU
-seconds). Five-fold cross-validation by chain: each chain is scored by a network that never saw
-it.
+Data: ``tensorforge_amd/tune/data/timings_mi355x.jsonl.gz``, the sweep of 20,000 random
+contraction chains (2 to 5 products, seed 2026, indices 0-19,999): every applicable candidate of
+the 7,921 chains that admit at least two candidate forms, timed on an MI355X in two measure
+passes whose per-candidate medians are averaged (``python -m tensorforge_amd.tune.sweep build |
+measure | pack``; the sweep's record, ``timings_mi355x.json``, holds the build's CPU seconds and
+each pass's GPU seconds). Five-fold cross-validation by chain: each chain is scored by a network
+that never saw it.
 
 The bars: the model's single pick (the mean of eight networks trained from different seeds) is
 within 5 % of the sweep's best on at least 90 % of the held-out chains, and beats the library's
@@ -32,7 +32,7 @@ def test_single_pick_within_5pct_on_90pct_of_held_out_chains(device):
   s = M.cross_validate(recs, folds=5, seed=0, prune_k=3, **SETTINGS)
   print(s)
   cands = len(recs) / s['chains']
-  assert s['chains'] >= 2000 and cands > 9
+  assert s['chains'] == 7921 and cands > 9
   assert s['model_pick']['within_5pct'] >= 0.90
   assert s['model_pick']['mean_slowdown'] < s['default']['mean_slowdown']
   assert s['model_pick']['within_5pct'] >= s['default']['within_5pct'] + 0.2
