@@ -11,6 +11,10 @@
 * ``method='exhaustive'``: every candidate is compiled and timed (the sweep; the fallback and the
   yardstick of the other two).
 
+Forms outside the database the model was trained on (``EXTRA``: the FP64 matrix-core form, for
+FP64 chains with dense shared operands) are never picked by the model; the timed methods time
+them whenever the chain admits them.
+
 Timing uses synthetic inputs of the chain's shapes at ``batch`` elements (median of up to 25
 launches, as the sweep times). Compiled candidates go to one shared object per chain in
 ``build_dir``; an object already there (same candidate kernel names) is reused, so a search can be
@@ -34,6 +38,11 @@ DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data', 'timings
 SETTINGS = dict(epochs=40, batch=2048, lr=0.004, ensemble=8, target='gap', cap=0.25)
 
 _MODELS = {}
+
+
+def _extra():
+  """Candidate forms the cost model was not trained on."""
+  return [space.Candidate('mfma_fp64', 'mfma_fp64', None)]
 
 
 @dataclass
@@ -70,25 +79,28 @@ def chain_kind(generator):
 
 
 def candidates(descrs, fp_type, batch, name_prefix):
-  """[(Candidate, Generator)] the chain admits and their feature records."""
+  """[(Candidate, Generator)] the chain admits, the feature records of the model's candidates
+  (the first ``len(records)`` entries) and, after them, the admitted forms of ``_extra()``."""
   cands = space.applicable(descrs, fp_type, name_prefix=name_prefix)
   if not cands:
     raise ValueError('no candidate form admits this description list')
   kind = chain_kind(cands[0][1])
   info = {'kind': kind}
   recs = [{'label': c.label, 'features': F.features(g, c, batch, info)} for c, g in cands]
-  return cands, recs
+  extra = space.applicable(descrs, fp_type, cands=_extra(), name_prefix=name_prefix)
+  return cands + extra, recs
 
 
 def _prefix(descrs, fp_type, batch):
   """A kernel-name prefix per (chain, type, batch): candidates of different chains loaded into one
-  process never share a symbol."""
+  process never share a symbol. The chain's kernel name alone does not separate chains that
+  differ only in a sparsity pattern, so the prefix also hashes the default form's source."""
   from ..common.context import Context
   from ..generators.generator import Generator
   g = Generator(descrs, Context('gfx950', 'hip', fp_type))
   g.generate()
-  h = hashlib.md5(f'{g.get_base_name()}/{fp_type}/{batch}'.encode()).hexdigest()[:10]
-  return f'at{h}'
+  h = hashlib.md5(f'{g.get_base_name()}/{fp_type}/{batch}/{g.get_kernel()}'.encode())
+  return f'at{h.hexdigest()[:10]}'
 
 
 def library(cands, build_dir):
@@ -97,7 +109,7 @@ def library(cands, build_dir):
   from ..runtime.build import compile_generators
   gens = [g for _, g in cands]
   name = gens[0].get_base_name() + '_' + hashlib.md5(
-    '/'.join(g.get_emitted().kernel_name for g in gens).encode()).hexdigest()[:8]
+    '/'.join(g.get_kernel() for g in gens).encode()).hexdigest()[:8]
   so = os.path.join(build_dir, f'lib{name}.so')
   if not os.path.exists(so):
     os.makedirs(build_dir, exist_ok=True)
@@ -144,8 +156,9 @@ def autotune(descrs, fp_type=FloatingPointType.DOUBLE, batch=65536, method='pick
     return Choice(labels[i], cands[i][0], cands[i][1], method, predicted)
   if build_dir is None:
     raise ValueError('the timed methods need build_dir for the compiled candidates')
-  idx = M.pruned(order, labels, k) if method == 'pruned' else list(range(len(cands)))
+  extra = list(range(len(recs), len(cands)))
+  idx = M.pruned(order, labels, k) + extra if method == 'pruned' else list(range(len(cands)))
   timed = _time(cands, idx, batch, build_dir, device, steps)
   best = min(timed, key=timed.get)
-  i = labels.index(best)
+  i = [c.label for c, _ in cands].index(best)
   return Choice(best, cands[i][0], cands[i][1], method, predicted, timed)

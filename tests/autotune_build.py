@@ -28,6 +28,9 @@ def chains():
     out.append((f'random_{i}', descrs, fp, batch))
   w = W.ader_volume(FloatingPointType.DOUBLE)
   out.append(('ader_f64', w.descrs, w.fp_type, 262144))
+  # dense K_d: the FP64 matrix-core form is admitted as well (timed, never predicted)
+  w = W.ader_volume(FloatingPointType.DOUBLE, sparse_k=False)
+  out.append(('ader_dense_f64', w.descrs, w.fp_type, 262144))
   return out
 
 

@@ -153,11 +153,20 @@ def test_autotune_chain_kind_and_candidate_records():
   assert kind(descrs, fp) == 'three_index'
   w = W.gemm()
   cands, recs = at.candidates(w.descrs, w.fp_type, 65536, 'attest')
-  assert len(cands) == len(recs) >= 2
+  assert len(recs) >= 2 and [c.label for c, _ in cands[len(recs):]] == ['mfma_fp64']
   assert all(len(r['features']) == len(F.NAMES) for r in recs)
   assert len({g.get_emitted().kernel_name for _, g in cands}) == len(cands)
   with pytest.raises(ValueError):
     at.autotune(w.descrs, w.fp_type, 65536, method='fastest')
+  # chains that differ only in a sparsity pattern share the reference's kernel name, not the
+  # autotuner's symbol prefix
+  sp, de = W.ader_volume(), W.ader_volume(sparse_k=False)
+  assert at._prefix(sp.descrs, sp.fp_type, 1000) != at._prefix(de.descrs, de.fp_type, 1000)
+  # the FP64 matrix-core form comes after the model's candidates, for dense chains only
+  cands, recs = at.candidates(de.descrs, de.fp_type, 262144, 'attest2')
+  assert [c.label for c, _ in cands[len(recs):]] == ['mfma_fp64']
+  cands, recs = at.candidates(sp.descrs, sp.fp_type, 262144, 'attest3')
+  assert len(cands) == len(recs)
 
 
 def test_sweep_pack_averages_passes_and_counts(tmp_path):
