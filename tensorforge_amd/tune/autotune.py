@@ -103,25 +103,25 @@ def _prefix(descrs, fp_type, batch):
   return f'at{h.hexdigest()[:10]}'
 
 
-def library(cands, build_dir):
-  """The shared object holding every candidate of one chain (compiled when missing): one object
-  serves every method, and can be built ahead of the GPU run."""
+def library(cands, build_dir, force=False):
+  """The shared object holding every candidate of one chain (compiled when missing, or always
+  with ``force``): one object serves every method, and can be built ahead of the GPU run."""
   from ..runtime.build import compile_generators
   gens = [g for _, g in cands]
   name = gens[0].get_base_name() + '_' + hashlib.md5(
     '/'.join(g.get_kernel() for g in gens).encode()).hexdigest()[:8]
   so = os.path.join(build_dir, f'lib{name}.so')
-  if not os.path.exists(so):
+  if force or not os.path.exists(so):
     os.makedirs(build_dir, exist_ok=True)
     compile_generators(gens, name, build_dir)
   return so
 
 
-def prepare(descrs, fp_type, batch, build_dir):
-  """Compile every candidate of the chain into ``build_dir`` (no GPU needed); returns the
-  object's path."""
+def prepare(descrs, fp_type, batch, build_dir, force=False):
+  """Compile every candidate of the chain into ``build_dir`` (no GPU needed; an existing object
+  is kept unless ``force``); returns the object's path."""
   cands, _ = candidates(descrs, fp_type, batch, _prefix(descrs, fp_type, batch))
-  return library(cands, build_dir)
+  return library(cands, build_dir, force)
 
 
 def _time(cands, idx, batch, build_dir, device, steps):

@@ -36,7 +36,7 @@ def chains():
 
 def build(directory=BUILD_DIR):
   from tensorforge_amd.tune.autotune import prepare
-  return [prepare(d, fp, batch, directory) for _, d, fp, batch in chains()]
+  return [prepare(d, fp, batch, directory, force=True) for _, d, fp, batch in chains()]
 
 
 if __name__ == '__main__':
