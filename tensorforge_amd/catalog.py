@@ -292,6 +292,22 @@ for _emitter in ('plain', 'tuned'):
 register('gemm_f64_mfma_fp64')(lambda: _wl_entry(W.gemm(F64), 'mfma_fp64'))
 register('ader_dense_f64_mfma_fp64')(
   lambda: _wl_entry(W.ader_volume(F64, sparse_k=False), 'mfma_fp64'))
+# its other shapes and destination placements (schedule knobs; each built and tested, none the
+# default)
+MFMA64_OPTIONS = {
+  'w8_k8_acc': dict(mfma_waves=8, mfma_k_block=8, mfma_dest='acc'),
+  'w8_k8_end': dict(mfma_waves=8, mfma_k_block=8, mfma_dest='end'),
+  'w4_k8_regs': dict(mfma_waves=4, mfma_k_block=8, mfma_dest='regs'),
+}
+
+
+def _mfma64_option(kw):
+  from .emit.tuned import Schedule
+  return _wl_entry(W.gemm(F64), 'mfma_fp64', Schedule(**kw))
+
+
+for _tag, _kw in MFMA64_OPTIONS.items():
+  register(f'opt_gemm_f64_mfma_{_tag}')(lambda kw=_kw: _mfma64_option(kw))
 
 # precision study: order 7 (84 basis functions), dense K_d, FP64 inputs and outputs
 for _emitter in ('plain', 'tuned', 'mfma_fp32', 'mfma_bf16x3'):

@@ -69,3 +69,28 @@ def test_fp64_mfma_many_rounds_equal_streaming_form(name, device):
   err = normwise_error(got, ref, entry.tensors[out].get_real_volume())
   print(f'{name} N={n}: normwise rel diff to the streaming form {err:.3e}')
   assert err <= 1e-12
+
+
+@pytest.mark.parametrize('tag', ['w8_k8_acc', 'w8_k8_end', 'w4_k8_regs'])
+def test_fp64_mfma_schedule_options(tag, device):
+  """The GEMM form's other shapes and destination placements (schedule knobs) against the
+  oracle, with flags and extra offsets, on a ragged batch."""
+  from tensorforge_amd import catalog
+  entry, kernel = catalog.load(f'opt_gemm_f64_mfma_{tag}')
+  sch = entry.generator.get_emitted().schedule
+  assert sch['kind'] == 'mfma_fp64'
+  assert (sch['waves_per_block'], sch['k_block']) == tuple(int(x[1:]) for x in tag.split('_')[:2])
+  n = 301
+  offsets = {'B': 3, 'C': 5}
+  host = make_inputs(entry, n, seed=12, offsets=offsets)
+  flags = (np.arange(n) % 5 != 2).astype(np.uint32)
+  ref = oracle_outputs(entry, host, n, offsets=offsets, flags=flags)['C']
+  got = gpu_run(kernel, entry, host, n, device, offsets=offsets, flags=flags)['C']
+  vol = entry.tensors['C'].get_real_volume()
+  assert np.array_equal(got[:5], host['C'][:5])
+  for e in range(n):
+    sl = slice(5 + e * vol, 5 + (e + 1) * vol)
+    if flags[e]:
+      assert normwise_error(got[sl], ref[sl], vol) <= 1e-12
+    else:
+      assert np.array_equal(got[sl], host['C'][sl])
