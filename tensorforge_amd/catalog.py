@@ -295,9 +295,9 @@ register('ader_dense_f64_mfma_fp64')(
 # its other shapes and destination placements (schedule knobs; each built and tested, none the
 # default)
 MFMA64_OPTIONS = {
-  'w8_k8_acc': dict(mfma_waves=8, mfma_k_block=8, mfma_dest='acc'),
-  'w8_k8_end': dict(mfma_waves=8, mfma_k_block=8, mfma_dest='end'),
-  'w4_k8_regs': dict(mfma_waves=4, mfma_k_block=8, mfma_dest='regs'),
+  'mfma64w8k8acc': dict(mfma_waves=8, mfma_k_block=8, mfma_dest='acc'),
+  'mfma64w8k8end': dict(mfma_waves=8, mfma_k_block=8, mfma_dest='end'),
+  'mfma64w4k8regs': dict(mfma_waves=4, mfma_k_block=8, mfma_dest='regs'),
 }
 
 
@@ -307,7 +307,7 @@ def _mfma64_option(kw):
 
 
 for _tag, _kw in MFMA64_OPTIONS.items():
-  register(f'opt_gemm_f64_mfma_{_tag}')(lambda kw=_kw: _mfma64_option(kw))
+  register(f'opt_gemm_f64_{_tag}')(lambda kw=_kw: _mfma64_option(kw))
 
 # precision study: order 7 (84 basis functions), dense K_d, FP64 inputs and outputs
 for _emitter in ('plain', 'tuned', 'mfma_fp32', 'mfma_bf16x3'):

@@ -71,15 +71,16 @@ def test_fp64_mfma_many_rounds_equal_streaming_form(name, device):
   assert err <= 1e-12
 
 
-@pytest.mark.parametrize('tag', ['w8_k8_acc', 'w8_k8_end', 'w4_k8_regs'])
-def test_fp64_mfma_schedule_options(tag, device):
+@pytest.mark.parametrize('tag,shape', [('mfma64w8k8acc', (8, 8)), ('mfma64w8k8end', (8, 8)),
+                                       ('mfma64w4k8regs', (4, 8))])
+def test_fp64_mfma_schedule_options(tag, shape, device):
   """The GEMM form's other shapes and destination placements (schedule knobs) against the
   oracle, with flags and extra offsets, on a ragged batch."""
   from tensorforge_amd import catalog
-  entry, kernel = catalog.load(f'opt_gemm_f64_mfma_{tag}')
+  entry, kernel = catalog.load(f'opt_gemm_f64_{tag}')
   sch = entry.generator.get_emitted().schedule
   assert sch['kind'] == 'mfma_fp64'
-  assert (sch['waves_per_block'], sch['k_block']) == tuple(int(x[1:]) for x in tag.split('_')[:2])
+  assert (sch['waves_per_block'], sch['k_block']) == shape
   n = 301
   offsets = {'B': 3, 'C': 5}
   host = make_inputs(entry, n, seed=12, offsets=offsets)
