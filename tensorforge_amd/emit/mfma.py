@@ -420,8 +420,8 @@ class _Mfma:
     return (f'{a.name} + {esafe} * {a.element_stride}ull + {a.name}_extraOffset + '
             f'(col + {c0 - X.origin[nd]}) * {X.strides[nd]}')
 
-  def load_rows(self, kb, dst, ptr='lq'):
-    """Rows of contraction block ``kb`` of the lane's column (at ``ptr``) into ``dst``."""
+  def load_rows(self, kb, dst):
+    """Rows of contraction block ``kb`` of the lane's column (at ``lq``) into ``dst``."""
     w = self.w
     pl = self.pl
     if self.B is not None:
@@ -432,7 +432,7 @@ class _Mfma:
     for j in range(self.KB):
       k = kb * self.KB + j
       if k < self.K:
-        w(f'{dst}[{j}] = {ptr}[{(k + pl.klo - origin) * stride}];')
+        w(f'{dst}[{j}] = lq[{(k + pl.klo - origin) * stride}];')
       else:
         w(f'{dst}[{j}] = 0.0;')
 
